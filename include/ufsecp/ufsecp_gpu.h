@@ -1,0 +1,142 @@
+/* ufsecp GPU batch C ABI -- MI355X (gfx950) implementation.
+ *
+ * Drop-in for the batch-verify / MSM part of the reference's
+ * include/ufsecp/ufsecp_gpu.h.  Every declaration below keeps the reference's
+ * name, argument order, strides, ownership and error behaviour; the
+ * reference location is cited on each entry point.  Conventions shared by all
+ * batch calls (reference src/cpu/src/ufsecp_gpu_impl.cpp:76-140, 317-558):
+ *
+ *   - ctx == NULL                      -> UFSECP_ERR_NULL_ARG
+ *   - count == 0                       -> UFSECP_OK, nothing written
+ *   - count > 2^26                     -> UFSECP_ERR_BAD_INPUT
+ *   - result buffers are zeroed before dispatch and again on any non-OK return
+ *     (the in-place *_collect calls excepted)
+ *   - a malformed item is a per-item 0 verdict, never an error code
+ *   - host buffers are owned by the caller; device memory is owned by ctx
+ *   - one ctx per thread, or external locking
+ *   - public data only: no secret key ever reaches the device.
+ *
+ * Backend id 4 ("ROCm") names this implementation; ids 1-3 (CUDA, OpenCL,
+ * Metal) are the reference's other backends and are not compiled here. */
+#ifndef UFSECP_GPU_H
+#define UFSECP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ufsecp_error.h"
+
+#ifdef __cplusplus
+#  define UFSECP_NOEXCEPT noexcept
+extern "C" {
+#else
+#  define UFSECP_NOEXCEPT
+#endif
+
+#define UFSECP_GPU_BACKEND_NONE   0
+#define UFSECP_GPU_BACKEND_CUDA   1
+#define UFSECP_GPU_BACKEND_OPENCL 2
+#define UFSECP_GPU_BACKEND_METAL  3
+#define UFSECP_GPU_BACKEND_ROCM   4
+
+typedef struct ufsecp_gpu_ctx ufsecp_gpu_ctx;
+
+/* Device description (reference ufsecp_gpu.h:121-129, same layout). */
+typedef struct {
+    char     name[128];
+    uint64_t global_mem_bytes;
+    uint32_t compute_units;
+    uint32_t max_clock_mhz;
+    uint32_t max_threads_per_block;
+    uint32_t backend_id;
+    uint32_t device_index;
+} ufsecp_gpu_device_info_t;
+
+/* ---- discovery and lifecycle (reference ufsecp_gpu.h:109-167) ----------- */
+
+/* Number of compiled-in backends; fills backend_ids[0..max_ids) if non-NULL.
+ * Replaces ufsecp_gpu_backend_count (ufsecp_gpu.h:109). */
+UFSECP_API uint32_t ufsecp_gpu_backend_count(uint32_t* backend_ids, uint32_t max_ids) UFSECP_NOEXCEPT;
+
+/* "ROCm" for 4, "CUDA"/"OpenCL"/"Metal" for 1-3, "none" otherwise
+ * (ufsecp_gpu.h:112). */
+UFSECP_API const char* ufsecp_gpu_backend_name(uint32_t backend_id);
+
+/* 1 iff the backend is compiled in and a device exists (ufsecp_gpu.h:115). */
+UFSECP_API int ufsecp_gpu_is_available(uint32_t backend_id) UFSECP_NOEXCEPT;
+
+/* Device count for a backend, 0 if unavailable (ufsecp_gpu.h:118). */
+UFSECP_API uint32_t ufsecp_gpu_device_count(uint32_t backend_id) UFSECP_NOEXCEPT;
+
+/* Fill *info_out for (backend, device) (ufsecp_gpu.h:132). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_device_info(uint32_t backend_id, uint32_t device_index,
+                                                 ufsecp_gpu_device_info_t* info_out);
+
+/* Create a context on one device; builds the device-resident G table
+ * (ufsecp_gpu.h:146). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ctx_create(ufsecp_gpu_ctx** ctx_out, uint32_t backend_id,
+                                                uint32_t device_index);
+
+/* Release a context and all its device memory; NULL is a no-op (ufsecp_gpu.h:152). */
+UFSECP_API void ufsecp_gpu_ctx_destroy(ufsecp_gpu_ctx* ctx);
+
+/* 1 while ctx accepts batch work, 0 for NULL (ufsecp_gpu.h:158). */
+UFSECP_API int ufsecp_gpu_is_ready(const ufsecp_gpu_ctx* ctx) UFSECP_NOEXCEPT;
+
+/* Last error code of ctx (ufsecp_gpu.h:161). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_last_error(const ufsecp_gpu_ctx* ctx);
+
+/* Last error message, borrowed until the next mutating call on ctx
+ * (ufsecp_gpu.h:167). */
+UFSECP_API const char* ufsecp_gpu_last_error_msg(const ufsecp_gpu_ctx* ctx);
+
+/* Static description of an error code (ufsecp_gpu.h:607). */
+UFSECP_API const char* ufsecp_gpu_error_str(ufsecp_error_t err);
+
+/* ---- batch verification ------------------------------------------------- */
+
+/* ECDSA: msg_hashes32 N*32, pubkeys33 N*33 (SEC1 compressed), sigs64 N*64
+ * compact big-endian r||s; out_results N bytes, 1 valid / 0 invalid.
+ * Verdicts equal the reference CPU ufsecp_ecdsa_verify: strict r,s in [1,n),
+ * low-S required, strict pubkey.  Replaces ufsecp_gpu_ecdsa_verify_batch
+ * (ufsecp_gpu.h:200-206). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdsa_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                        const uint8_t* pubkeys33, const uint8_t* sigs64,
+                                                        size_t count, uint8_t* out_results);
+
+/* BIP-340: msg_hashes32 N*32, pubkeys_x32 N*32, sigs64 N*64 (r||s);
+ * out_results N bytes.  Verdicts equal the reference CPU
+ * ufsecp_schnorr_verify (r < p, 0 < s < n, x-only key < p).  Replaces
+ * ufsecp_gpu_schnorr_verify_batch (ufsecp_gpu.h:254-260). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_schnorr_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                          const uint8_t* pubkeys_x32, const uint8_t* sigs64,
+                                                          size_t count, uint8_t* out_results);
+
+/* In-place verdict collapse: key_buffer bytes are caller-seeded non-zero
+ * markers; a VALID item zeroes its byte, an invalid one leaves it.  Not
+ * pre-cleared.  Replaces ufsecp_gpu_ecdsa_verify_collect (ufsecp_gpu.h:326-332). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdsa_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                          const uint8_t* pubkeys33, const uint8_t* sigs64,
+                                                          size_t count, uint8_t* key_buffer);
+
+/* Schnorr sibling of the above.  Replaces ufsecp_gpu_schnorr_verify_collect
+ * (ufsecp_gpu.h:345-351). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_schnorr_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                            const uint8_t* pubkeys_x32, const uint8_t* sigs64,
+                                                            size_t count, uint8_t* key_buffer);
+
+/* ---- multi-scalar multiplication ---------------------------------------- */
+
+/* out_result33 = compressed sum(scalars[i] * points[i]); scalars32 N*32
+ * big-endian, points33 N*33 compressed.  Scalars >= n -> BAD_INPUT, invalid
+ * point -> BAD_PUBKEY (the reference CPU ufsecp_multi_scalar_mul semantics,
+ * impl/ufsecp_taproot.cpp:957-991); result at infinity -> ERR_ARITH with the
+ * output zeroed.  Replaces ufsecp_gpu_msm (ufsecp_gpu.h:397-402). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_msm(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32, const uint8_t* points33,
+                                         size_t n, uint8_t* out_result33);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UFSECP_GPU_H */

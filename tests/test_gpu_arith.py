@@ -1,0 +1,122 @@
+"""Device field / scalar / GLV / point primitives against textbook integer math.
+
+Each op runs in the HIP self-test kernel (``ufsecp_gpu_ext_selftest``) on
+random values plus the edge values a limb implementation gets wrong
+(0, 1, p-1, p, 2^256-1, values just above p, carries across every limb).
+"""
+import random
+
+import numpy as np
+import pytest
+
+from ec_math import BETA, G, LAMBDA, N, P, add, from_limbs, lift_x, mul, to_limbs
+
+pytestmark = pytest.mark.gpu
+
+OP_FE_MUL, OP_FE_SQR, OP_FE_ADD, OP_FE_SUB, OP_FE_INV, OP_FE_SQRT = 0, 1, 2, 3, 4, 5
+OP_SC_MUL, OP_SC_INV, OP_GLV = 10, 11, 12
+OP_GEN_MUL, OP_VAR_MUL = 20, 21
+
+M256 = (1 << 256) - 1
+EDGE = [0, 1, 2, P - 1, P, P + 1, M256, M256 - 1, (1 << 255), (1 << 32) - 1, 977, 0x1000003D1,
+        (1 << 224) - 1, P - 0x1000003D1, N - 1, N, N + 1] + [((1 << 32) - 1) << (32 * i) for i in range(8)]
+
+
+def _vals(n, seed, edge=EDGE):
+    rnd = random.Random(seed)
+    vals = list(edge) + [rnd.getrandbits(256) for _ in range(n)]
+    return vals
+
+
+def _pack(vals):
+    a = np.zeros((len(vals), 16), np.uint32)
+    for i, v in enumerate(vals):
+        a[i, :8] = to_limbs(v)
+    return a
+
+
+def test_field_mul_sqr_add_sub(gpu):
+    xs = _vals(300, 1)
+    ys = list(reversed(_vals(300, 2)))
+    a, b = _pack(xs), _pack(ys)
+    for op, f in ((OP_FE_MUL, lambda x, y: x * y % P), (OP_FE_SQR, lambda x, y: x * x % P),
+                  (OP_FE_ADD, lambda x, y: (x + y) % P), (OP_FE_SUB, lambda x, y: (x - y) % P)):
+        out = gpu.selftest(op, a, b)
+        for i, (x, y) in enumerate(zip(xs, ys)):
+            assert from_limbs(out[i, :8]) == f(x, y), (op, hex(x), hex(y))
+
+
+def test_field_inv_sqrt(gpu):
+    xs = [v for v in _vals(100, 3) if v % P != 0]
+    a = _pack(xs)
+    out = gpu.selftest(OP_FE_INV, a, a)
+    for i, x in enumerate(xs):
+        assert from_limbs(out[i, :8]) == pow(x, -1, P), hex(x)
+    out = gpu.selftest(OP_FE_SQRT, a, a)
+    for i, x in enumerate(xs):
+        r = pow(x % P, (P + 1) // 4, P)
+        ok = r * r % P == x % P
+        assert int(out[i, 8]) == int(ok)
+        assert from_limbs(out[i, :8]) == r
+
+
+def test_scalar_mul_inv(gpu):
+    xs = [v % N for v in _vals(200, 4)]
+    ys = [v % N for v in _vals(200, 5)][::-1]
+    out = gpu.selftest(OP_SC_MUL, _pack(xs), _pack(ys))
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        assert from_limbs(out[i, :8]) == x * y % N
+    nz = [x for x in xs if x]
+    out = gpu.selftest(OP_SC_INV, _pack(nz), _pack(nz))
+    for i, x in enumerate(nz):
+        assert from_limbs(out[i, :8]) == pow(x, -1, N)
+
+
+def test_glv_split(gpu):
+    ks = [v % N for v in _vals(300, 6)] + [LAMBDA, N - LAMBDA, (N - 1) // 2, (N + 1) // 2]
+    out = gpu.selftest(OP_GLV, _pack(ks), _pack(ks))
+    for i, k in enumerate(ks):
+        k1, k2 = from_limbs(out[i, :8]), from_limbs(out[i, 8:16])
+        n1, n2 = int(out[i, 16]), int(out[i, 17])
+        assert k1 < 2 ** 128 and k2 < 2 ** 128
+        s1 = -k1 if n1 else k1
+        s2 = -k2 if n2 else k2
+        assert (s1 + s2 * LAMBDA - k) % N == 0
+
+
+def test_generator_mul(gpu):
+    rnd = random.Random(7)
+    ks = [1, 2, 3, N - 1, N - 2, (N - 1) // 2, 0, 1 << 255, (1 << 128) + 1] + [rnd.randrange(N) for _ in range(60)]
+    out = gpu.selftest(OP_GEN_MUL, _pack(ks), _pack(ks))
+    for i, k in enumerate(ks):
+        exp = mul(k)
+        if exp is None:
+            assert int(out[i, 16]) == 1
+        else:
+            assert int(out[i, 16]) == 0
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, k
+
+
+def test_variable_mul_glv(gpu):
+    rnd = random.Random(8)
+    xs, ks, pts = [], [], []
+    while len(xs) < 40:
+        x = rnd.getrandbits(256) % P
+        pt = lift_x(x)
+        if pt is None:
+            continue
+        xs.append(x)
+        pts.append(pt)
+        ks.append(rnd.randrange(N))
+    ks[0], ks[1], ks[2], ks[3] = 0, 1, N - 1, LAMBDA
+    out = gpu.selftest(OP_VAR_MUL, _pack(xs), _pack(ks))
+    for i, (pt, k) in enumerate(zip(pts, ks)):
+        assert int(out[i, 17]) == 1
+        exp = mul(k, pt)
+        if exp is None:
+            assert int(out[i, 16]) == 1
+        else:
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
+    # lambda * P == (beta * x, y)
+    assert mul(LAMBDA, pts[3]) == (BETA * pts[3][0] % P, pts[3][1])
+    _ = (G, add)

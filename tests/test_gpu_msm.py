@@ -1,0 +1,49 @@
+"""Pippenger MSM on the GPU vs the reference CPU ``ufsecp_multi_scalar_mul``.
+
+Small cases and error precedence come from tests/golden/edge_cases.json (the
+reference CPU ABI's outputs and codes).  Larger MSMs use bases k_i*G, whose
+result has the closed form (sum s_i k_i mod n) * G.
+"""
+import numpy as np
+import pytest
+
+from ec_math import N, compress, mul
+from ultrafastsecp256k1_amd import UFSECP_ERR_ARITH, UFSECP_OK
+from ultrafastsecp256k1_amd import workload as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def test_msm_golden_cases(gpu, golden):
+    for c in golden["edge_cases"]["msm"]:
+        rc, out = gpu.msm_rc(bytes.fromhex(c["scalars"]), bytes.fromhex(c["points"]), c["n"])
+        assert rc == c["rc"], c["name"]
+        if rc == UFSECP_OK:
+            assert out.hex() == c["result"], c["name"]
+        else:
+            assert out == b"\x00" * 33, c["name"]
+
+
+def test_msm_workload_4096_matches_reference(gpu, golden):
+    f = golden["workload_heads"]["msm"]
+    sc, pts, _ = wl.generate_msm(f["n"], f["seed"])
+    assert gpu.msm(sc, pts).hex() == f["result"]
+
+
+@pytest.mark.parametrize("n", [1 << 12, 1 << 16])
+def test_msm_closed_form(gpu, n):
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 77)
+    e = wl.msm_closed_form_scalar(sc, ks)
+    assert gpu.msm(sc, pts) == compress(mul(e))
+
+
+def test_msm_infinity_is_arith_error(gpu):
+    sc, pts, ks = wl.generate_msm(64, seed=5)
+    sc2 = sc.copy()
+    # s_63 chosen so that the total is 0 mod n
+    e = wl.msm_closed_form_scalar(sc[:63], ks[:63])
+    k63 = int.from_bytes(bytes(ks[63]), "big")
+    s63 = (-e) * pow(k63, -1, N) % N
+    sc2[63] = np.frombuffer(s63.to_bytes(32, "big"), np.uint8)
+    rc, out = gpu.msm_rc(sc2, pts)
+    assert rc == UFSECP_ERR_ARITH and out == b"\x00" * 33

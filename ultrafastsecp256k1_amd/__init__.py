@@ -1,0 +1,239 @@
+"""MI355X-native secp256k1 batch verification and MSM.
+
+Python host-side mirror of the ufsecp GPU C ABI (``include/ufsecp/ufsecp_gpu.h``)
+over ``libufsecp_gpu.so``, in the style of the reference's ctypes wrapper
+(bindings/python/ufsecp/__init__.py:643-834, class ``UfsecpGpu``).  Everything
+here is plumbing: the arithmetic runs in the HIP kernels of the library, and
+the wrapper raises if the library is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, byref, c_int, c_size_t, c_uint32, c_void_p, c_char_p, c_uint8
+from pathlib import Path
+from typing import Optional
+
+PKG_DIR = Path(__file__).resolve().parent
+GPU_LIB_PATH = PKG_DIR / "libufsecp_gpu.so"
+WORKLOAD_LIB_PATH = PKG_DIR / "libufsecp_workload.so"
+
+UFSECP_OK = 0
+UFSECP_ERR_NULL_ARG = 1
+UFSECP_ERR_BAD_PUBKEY = 3
+UFSECP_ERR_BAD_INPUT = 5
+UFSECP_ERR_ARITH = 7
+UFSECP_ERR_INTERNAL = 9
+UFSECP_ERR_GPU_UNAVAILABLE = 100
+UFSECP_ERR_GPU_DEVICE = 101
+UFSECP_ERR_GPU_MEMORY = 103
+UFSECP_ERR_GPU_UNSUPPORTED = 104
+
+BACKEND_ROCM = 4
+
+# exported symbols of libufsecp_gpu.so, as declared in include/ufsecp/*.h
+ABI_SYMBOLS = (
+    "ufsecp_gpu_backend_count", "ufsecp_gpu_backend_name", "ufsecp_gpu_is_available",
+    "ufsecp_gpu_device_count", "ufsecp_gpu_device_info", "ufsecp_gpu_ctx_create",
+    "ufsecp_gpu_ctx_destroy", "ufsecp_gpu_is_ready", "ufsecp_gpu_last_error",
+    "ufsecp_gpu_last_error_msg", "ufsecp_gpu_error_str", "ufsecp_gpu_ecdsa_verify_batch",
+    "ufsecp_gpu_schnorr_verify_batch", "ufsecp_gpu_ecdsa_verify_collect",
+    "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_msm",
+    "ufsecp_gpu_ext_ecdsa_verify_device", "ufsecp_gpu_ext_schnorr_verify_device",
+    "ufsecp_gpu_ext_stream", "ufsecp_gpu_ext_synchronize", "ufsecp_gpu_ext_selftest",
+)
+
+
+class UfsecpError(RuntimeError):
+    def __init__(self, op: str, code: int, msg: str = ""):
+        self.op = op
+        self.code = code
+        super().__init__(f"{op} failed: code {code}{(': ' + msg) if msg else ''}")
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load libufsecp_gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path or os.environ.get("UFSECP_GPU_LIB") or GPU_LIB_PATH)
+    if not p.is_file():
+        raise FileNotFoundError(
+            f"{p} not found: build it with `python -m ultrafastsecp256k1_amd.build`")
+    lib = ctypes.CDLL(str(p))
+    _bind(lib)
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _bind(lib: ctypes.CDLL) -> None:
+    u8p = POINTER(c_uint8)
+    vp = c_void_p
+    sig = {
+        "ufsecp_gpu_backend_count": (c_uint32, [POINTER(c_uint32), c_uint32]),
+        "ufsecp_gpu_backend_name": (c_char_p, [c_uint32]),
+        "ufsecp_gpu_is_available": (c_int, [c_uint32]),
+        "ufsecp_gpu_device_count": (c_uint32, [c_uint32]),
+        "ufsecp_gpu_ctx_create": (c_int, [POINTER(vp), c_uint32, c_uint32]),
+        "ufsecp_gpu_ctx_destroy": (None, [vp]),
+        "ufsecp_gpu_is_ready": (c_int, [vp]),
+        "ufsecp_gpu_last_error": (c_int, [vp]),
+        "ufsecp_gpu_last_error_msg": (c_char_p, [vp]),
+        "ufsecp_gpu_error_str": (c_char_p, [c_int]),
+        "ufsecp_gpu_ecdsa_verify_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_schnorr_verify_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ecdsa_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_schnorr_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ext_ecdsa_verify_device": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
+        "ufsecp_gpu_ext_schnorr_verify_device": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
+        "ufsecp_gpu_ext_stream": (vp, [vp]),
+        "ufsecp_gpu_ext_synchronize": (c_int, [vp]),
+        "ufsecp_gpu_ext_selftest": (c_int, [vp, c_int, vp, vp, vp, c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _ = u8p
+
+
+def _buf(b) -> c_void_p:
+    """Pointer to the start of a bytes-like / numpy buffer (None -> NULL)."""
+    if b is None:
+        return c_void_p(None)
+    if isinstance(b, int):
+        return c_void_p(b)
+    try:
+        import numpy as np
+        if isinstance(b, np.ndarray):
+            return c_void_p(b.ctypes.data)
+    except ImportError:  # pragma: no cover
+        pass
+    if isinstance(b, bytes):
+        return ctypes.cast(ctypes.c_char_p(b), c_void_p)
+    if isinstance(b, (bytearray, memoryview)):
+        arr = (ctypes.c_char * len(b)).from_buffer(b)
+        return ctypes.cast(arr, c_void_p)
+    raise TypeError(f"unsupported buffer type {type(b)}")
+
+
+class UfsecpGpu:
+    """One GPU context (backend 4 = ROCm) with the batch operations of the ABI.
+
+    Usage::
+
+        with UfsecpGpu() as gpu:
+            verdicts = gpu.schnorr_verify_batch(msgs, xonly, sigs)
+    """
+
+    def __init__(self, device_index: int = 0, backend_id: int = BACKEND_ROCM,
+                 lib_path: Optional[str] = None):
+        self._lib = load_library(lib_path)
+        ctx = c_void_p()
+        rc = self._lib.ufsecp_gpu_ctx_create(byref(ctx), c_uint32(backend_id), c_uint32(device_index))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ctx_create", rc)
+        self._ctx = ctx
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.ufsecp_gpu_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *_):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self) -> c_void_p:
+        return self._ctx
+
+    @property
+    def lib(self) -> ctypes.CDLL:
+        return self._lib
+
+    def last_error_msg(self) -> str:
+        return self._lib.ufsecp_gpu_last_error_msg(self._ctx).decode()
+
+    def _check(self, op: str, rc: int) -> None:
+        if rc != UFSECP_OK:
+            raise UfsecpError(op, rc, self.last_error_msg())
+
+    # -- batch verification ---------------------------------------------------
+    def ecdsa_verify_batch(self, msgs, pubkeys33, sigs64, count: Optional[int] = None):
+        import numpy as np
+        n = len(msgs) // 32 if count is None else count
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_ecdsa_verify_batch(self._ctx, _buf(msgs), _buf(pubkeys33), _buf(sigs64), n,
+                                                     _buf(out))
+        self._check("ufsecp_gpu_ecdsa_verify_batch", rc)
+        return out
+
+    def schnorr_verify_batch(self, msgs, pubkeys_x32, sigs64, count: Optional[int] = None):
+        import numpy as np
+        n = len(msgs) // 32 if count is None else count
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_schnorr_verify_batch(self._ctx, _buf(msgs), _buf(pubkeys_x32), _buf(sigs64), n,
+                                                       _buf(out))
+        self._check("ufsecp_gpu_schnorr_verify_batch", rc)
+        return out
+
+    def msm(self, scalars32, points33, n: Optional[int] = None) -> bytes:
+        n = len(scalars32) // 32 if n is None else n
+        out = ctypes.create_string_buffer(33)
+        rc = self._lib.ufsecp_gpu_msm(self._ctx, _buf(scalars32), _buf(points33), n, out)
+        self._check("ufsecp_gpu_msm", rc)
+        return out.raw
+
+    def msm_rc(self, scalars32, points33, n: Optional[int] = None):
+        """(return code, 33-byte output) without raising."""
+        n = len(scalars32) // 32 if n is None else n
+        out = ctypes.create_string_buffer(33)
+        rc = self._lib.ufsecp_gpu_msm(self._ctx, _buf(scalars32), _buf(points33), n, out)
+        return rc, out.raw
+
+    # -- device-resident variants (pointers are device addresses) --------------
+    def schnorr_verify_device(self, d_msgs: int, d_pubx: int, d_sigs: int, count: int, d_out: int,
+                              stream: Optional[int] = None) -> None:
+        rc = self._lib.ufsecp_gpu_ext_schnorr_verify_device(self._ctx, c_void_p(d_msgs), c_void_p(d_pubx),
+                                                            c_void_p(d_sigs), count, c_void_p(d_out),
+                                                            c_void_p(stream))
+        self._check("ufsecp_gpu_ext_schnorr_verify_device", rc)
+
+    def ecdsa_verify_device(self, d_msgs: int, d_pubs: int, d_sigs: int, count: int, d_out: int,
+                            stream: Optional[int] = None) -> None:
+        rc = self._lib.ufsecp_gpu_ext_ecdsa_verify_device(self._ctx, c_void_p(d_msgs), c_void_p(d_pubs),
+                                                          c_void_p(d_sigs), count, c_void_p(d_out),
+                                                          c_void_p(stream))
+        self._check("ufsecp_gpu_ext_ecdsa_verify_device", rc)
+
+    def synchronize(self) -> None:
+        self._check("ufsecp_gpu_ext_synchronize", self._lib.ufsecp_gpu_ext_synchronize(self._ctx))
+
+    def stream(self) -> int:
+        return self._lib.ufsecp_gpu_ext_stream(self._ctx) or 0
+
+    def selftest(self, op: int, a, b):
+        """Run device arithmetic op on uint32 arrays a, b of shape (N, 16)."""
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        n = a.shape[0]
+        out = np.zeros((n, 32), dtype=np.uint32)
+        rc = self._lib.ufsecp_gpu_ext_selftest(self._ctx, op, _buf(a), _buf(b), _buf(out), n)
+        self._check("ufsecp_gpu_ext_selftest", rc)
+        return out

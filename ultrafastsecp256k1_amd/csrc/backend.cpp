@@ -1,0 +1,223 @@
+// RocmBackend implementation (host side).  See backend.hpp.
+#include "backend.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace ufs {
+
+hipError_t DeviceBuffer::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = 1;
+    while (want < bytes) want <<= 1;   // grow-only, power of two
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        cap = 0;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+hipError_t PinnedBuffer::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = 1;
+    while (want < bytes) want <<= 1;
+    hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        cap = 0;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void PinnedBuffer::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+int RocmBackend::device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+GpuError RocmBackend::device_info(uint32_t device_index, DeviceInfoData& out) {
+    const int n = device_count();
+    if (n <= 0) return GpuError::Unavailable;
+    if ((int)device_index >= n) return GpuError::Device;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, (int)device_index) != hipSuccess) return GpuError::Device;
+    std::memset(&out, 0, sizeof(out));
+    std::snprintf(out.name, sizeof(out.name), "%s (%s)", p.name, p.gcnArchName);
+    out.global_mem_bytes = p.totalGlobalMem;
+    out.compute_units = (uint32_t)p.multiProcessorCount;
+    out.max_clock_mhz = (uint32_t)(p.clockRate / 1000);
+    out.max_threads_per_block = (uint32_t)p.maxThreadsPerBlock;
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::fail(GpuError e, const char* what, hipError_t he) {
+    last_err_ = e;
+    last_msg_ = what;
+    if (he != hipSuccess) {
+        last_msg_ += ": ";
+        last_msg_ += hipGetErrorString(he);
+    }
+    return e;
+}
+
+RocmBackend::~RocmBackend() {
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& b : d_in_) b.release();
+    d_out_.release();
+    h_stage_.release();
+    if (d_gcomb_) (void)hipFree(d_gcomb_);
+    if (d_mid_) (void)hipFree(d_mid_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+static int env_int(const char* name, int dflt, int lo, int hi) {
+    const char* s = std::getenv(name);
+    if (!s || !*s) return dflt;
+    int v = std::atoi(s);
+    if (v < lo || v > hi) return dflt;
+    return v;
+}
+
+GpuError RocmBackend::init(uint32_t device_index) {
+    const int n = device_count();
+    if (n <= 0) return fail(GpuError::Unavailable, "no HIP device");
+    if ((int)device_index >= n) return fail(GpuError::Device, "device index out of range");
+    hipError_t e = hipSetDevice((int)device_index);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    device_ = (int)device_index;
+    e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(GpuError::Queue, "hipStreamCreate", e);
+
+    // Fixed-base comb table for G: window width from UFSECP_GCOMB_W (4..12).
+    const int w = env_int("UFSECP_GCOMB_W", 8, 4, 12);
+    const size_t entries = gcomb_entries(w);
+    e = hipMalloc(&d_gcomb_, entries * 64);
+    if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);
+    e = launch_build_gcomb(d_gcomb_, w, gcomb_nwin(w), stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "build G table", e);
+    gdesc_ = GcombDesc{d_gcomb_, w, gcomb_nwin(w)};
+
+    e = hipMalloc((void**)&d_mid_, 8 * sizeof(uint32_t));
+    if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(midstate)", e);
+    e = launch_bip340_midstate(d_mid_, stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "BIP-340 midstate", e);
+    e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return fail(GpuError::Backend, "init synchronize", e);
+    ready_ = true;
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::synchronize() {
+    hipError_t e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return fail(GpuError::Backend, "hipStreamSynchronize", e);
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
+                                  size_t count, uint8_t* out, bool collect) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    hipError_t e = hipSetDevice(device_);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    const size_t pub_stride = schnorr ? 32 : 33;
+    const size_t sz_m = count * 32, sz_p = count * pub_stride, sz_s = count * 64;
+    if ((e = d_in_[0].ensure(sz_m)) != hipSuccess || (e = d_in_[1].ensure(sz_p)) != hipSuccess ||
+        (e = d_in_[2].ensure(sz_s)) != hipSuccess || (e = d_out_.ensure(count)) != hipSuccess)
+        return fail(GpuError::Memory, "hipMalloc(batch buffers)", e);
+    e = hipMemcpyAsync(d_in_[0].ptr, msgs, sz_m, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[1].ptr, pubs, sz_p, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[2].ptr, sigs, sz_s, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess && collect)
+        e = hipMemcpyAsync(d_out_.ptr, out, count, hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy inputs to device", e);
+    const uint8_t* dm = (const uint8_t*)d_in_[0].ptr;
+    const uint8_t* dp = (const uint8_t*)d_in_[1].ptr;
+    const uint8_t* ds = (const uint8_t*)d_in_[2].ptr;
+    uint8_t* dout = (uint8_t*)d_out_.ptr;
+    if (schnorr)
+        e = launch_schnorr_verify(dm, dp, ds, (uint32_t)count, gdesc_, d_mid_, dout, collect, stream_);
+    else
+        e = launch_ecdsa_verify(dm, dp, ds, (uint32_t)count, gdesc_, dout, collect, stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "verify kernel launch", e);
+    e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
+    e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return fail(GpuError::Backend, "verify synchronize", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::ecdsa_verify_batch(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs,
+                                         size_t count, uint8_t* out, bool collect) {
+    return verify_host(false, msgs, pubs33, sigs, count, out, collect);
+}
+
+GpuError RocmBackend::schnorr_verify_batch(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs,
+                                           size_t count, uint8_t* out, bool collect) {
+    return verify_host(true, msgs, pubx, sigs, count, out, collect);
+}
+
+GpuError RocmBackend::ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,
+                                          size_t count, uint8_t* d_out, hipStream_t st) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    hipError_t e = launch_ecdsa_verify(d_msgs, d_pubs33, d_sigs, (uint32_t)count, gdesc_, d_out, false,
+                                       st ? st : stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "ECDSA verify launch", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::schnorr_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubx, const uint8_t* d_sigs,
+                                            size_t count, uint8_t* d_out, hipStream_t st) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    hipError_t e = launch_schnorr_verify(d_msgs, d_pubx, d_sigs, (uint32_t)count, gdesc_, d_mid_, d_out, false,
+                                         st ? st : stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "Schnorr verify launch", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t count) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    DeviceBuffer da, db, dout;
+    hipError_t e;
+    if ((e = da.ensure(count * 64)) != hipSuccess || (e = db.ensure(count * 64)) != hipSuccess ||
+        (e = dout.ensure(count * 128)) != hipSuccess) {
+        da.release(); db.release(); dout.release();
+        return fail(GpuError::Memory, "selftest buffers", e);
+    }
+    e = hipMemcpyAsync(da.ptr, a, count * 64, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(db.ptr, b, count * 64, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess)
+        e = launch_selftest(op, (const uint32_t*)da.ptr, (const uint32_t*)db.ptr, (uint32_t*)dout.ptr,
+                            (uint32_t)count, gdesc_, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout.ptr, count * 128, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    da.release(); db.release(); dout.release();
+    if (e != hipSuccess) return fail(GpuError::Backend, "selftest", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+}  // namespace ufs

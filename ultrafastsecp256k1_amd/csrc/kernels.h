@@ -1,0 +1,48 @@
+// Host-visible declarations of the HIP kernel launchers.  Included by the
+// kernel translation units and by the host backend; contains no device code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ufs {
+
+constexpr int kVerifyBlock = 256;
+
+// Fixed-base comb table for G resident in device memory.
+struct GcombDesc {
+    const void* table;   // (nwin * 2^(w-1) + 1) affine points, 64 B each
+    int w;
+    int nwin;
+};
+
+inline int gcomb_nwin(int w) { return (256 + w - 1) / w; }
+inline size_t gcomb_entries(int w) { return (size_t)gcomb_nwin(w) * ((size_t)1 << (w - 1)) + 1; }
+
+// Self-test operation codes for launch_selftest (inputs are 16-word records,
+// outputs 32-word records; see k_selftest).
+enum SelftestOp : int {
+    kOpFeMul = 0,
+    kOpFeSqr = 1,
+    kOpFeAdd = 2,
+    kOpFeSub = 3,
+    kOpFeInv = 4,
+    kOpFeSqrt = 5,
+    kOpScMul = 10,
+    kOpScInv = 11,
+    kOpGlv = 12,
+    kOpGenMul = 20,
+    kOpVarMul = 21,
+};
+
+hipError_t launch_ecdsa_verify(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs, uint32_t count,
+                               const GcombDesc& g, uint8_t* out, bool collect, hipStream_t st);
+hipError_t launch_schnorr_verify(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, uint32_t count,
+                                 const GcombDesc& g, const uint32_t* mid, uint8_t* out, bool collect,
+                                 hipStream_t st);
+hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
+hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
+hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,
+                           const GcombDesc& g, hipStream_t st);
+
+}  // namespace ufs

@@ -1,0 +1,352 @@
+// Batch ECDSA / BIP-340 verification kernels for gfx950 (one lane per
+// signature) plus context set-up kernels (fixed-base G comb table, BIP-340
+// challenge midstate) and arithmetic self-test kernels.
+//
+// Per-item semantics follow the reference CPU path, which is the parity
+// contract:
+//   ECDSA  : src/cpu/src/impl/ufsecp_ecdsa.cpp:244-275 (strict r,s in [1,n),
+//            low-S required), ecdsa.cpp:891-927 (strict compressed pubkey),
+//            ecdsa.cpp:764-850 (x(R) == r, including the r + n case).
+//   Schnorr: src/cpu/src/schnorr.cpp:355-371 (r < p, 0 < s < n),
+//            :275-300 (x-only key x < p, lift_x), :745-806 (R = sG - eP,
+//            R finite, even y, x(R) == r).
+#include "kernels.h"
+#include "secp_ecmult.h"
+#include "secp_hash.h"
+
+namespace ufs {
+
+UFS_DEV void load_be32_aligned(u32 r[8], const u8* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    const uint4 a = q[0], b = q[1];
+    const u32 w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = bswap32(w[7 - i]);
+}
+
+UFS_DEV void load_be_words(u32 w[8], const u8* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    const uint4 a = q[0], b = q[1];
+    w[0] = bswap32(a.x); w[1] = bswap32(a.y); w[2] = bswap32(a.z); w[3] = bswap32(a.w);
+    w[4] = bswap32(b.x); w[5] = bswap32(b.y); w[6] = bswap32(b.z); w[7] = bswap32(b.w);
+}
+
+UFS_DEV void load_be32_bytes(u32 r[8], const u8* p) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const u8* q = p + 28 - 4 * i;
+        r[i] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
+    }
+}
+
+UFS_DEV void set_generator(aff& g) {
+    const fe GX = {{UFS_GX_LIMBS}};
+    const fe GY = {{UFS_GY_LIMBS}};
+    g.x = GX;
+    g.y = GY;
+}
+
+// x(R) == r (mod n) test without inversion: r*Z^2 == X, or (r+n)*Z^2 == X
+// when r + n < p.
+UFS_DEV bool ecdsa_xcheck(const jac& R, const u32 r[8]) {
+    fe rz, z2, t;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rz.v[i] = r[i];
+    fe_sqr(z2, R.z);
+    fe_mul(t, rz, z2);
+    bool ok = fe_equal(t, R.x);
+    const u32 PMN[8] = {UFS_PMN_LIMBS};
+    const u32 N[8] = {UFS_N_LIMBS};
+    if (u256_lt(r, PMN)) {
+        u32 c = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) rz.v[i] = addc(r[i], N[i], c, c);
+        fe_mul(t, rz, z2);
+        ok |= fe_equal(t, R.x);
+    }
+    return ok;
+}
+
+// One ECDSA verification; returns the verdict.
+UFS_DEV bool ecdsa_verify_one(const u32 z_raw[8], const u32 r_raw[8], const u32 s_raw[8],
+                              u32 prefix, const u32 x_raw[8], const gcomb& G) {
+    bool ok = !u256_is_zero(r_raw) && u256_lt_n(r_raw) && !u256_is_zero(s_raw) && u256_lt_n(s_raw);
+    sc s, r, z;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { s.v[i] = s_raw[i]; r.v[i] = r_raw[i]; z.v[i] = z_raw[i]; }
+    ok &= !sc_is_high(s);
+    ok &= (prefix == 2u) | (prefix == 3u);
+    aff P;
+    ok &= lift_x(P, x_raw, prefix & 1u);
+    if (!ok) {
+        // keep the lane on the common path with harmless values
+        set_generator(P);
+        sc_set_u32(s, 1);
+        sc_set_u32(r, 1);
+    }
+    sc_reduce_once(z);
+    sc w, u1, u2;
+    sc_inv(w, s);
+    sc_mul(u1, z, w);
+    sc_mul(u2, r, w);
+    sc k1, k2;
+    bool n1, n2;
+    glv_split(k1, k2, n1, n2, u2);
+    ptable T;
+    build_ptable(T, P);
+    jac RP, RG, R;
+    ecmult_var(RP, T, k1, n1, k2, n2);
+    ecmult_gen(RG, G, u1);
+    jac_add(R, RG, RP);
+    const bool v = !R.inf && ecdsa_xcheck(R, r.v);
+    return ok && v;
+}
+
+// One BIP-340 verification; all inputs are big-endian words.
+UFS_DEV bool schnorr_verify_one(const u32 m_w[8], const u32 px_w[8], const u32 r_w[8], const u32 s_w[8],
+                                const u32 mid[8], const gcomb& G) {
+    u32 px[8], rr[8], ss[8];
+    u256_from_be_words(px, px_w);
+    u256_from_be_words(rr, r_w);
+    u256_from_be_words(ss, s_w);
+    bool ok = u256_lt_p(rr) && !u256_is_zero(ss) && u256_lt_n(ss);
+    aff P;
+    ok &= lift_x(P, px, 0u);
+    sc s;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s.v[i] = ss[i];
+    if (!ok) {
+        set_generator(P);
+        sc_set_u32(s, 1);
+    }
+    u32 ew[8];
+    bip340_challenge(ew, mid, r_w, px_w, m_w);
+    sc e, b;
+    u256_from_be_words(e.v, ew);
+    sc_reduce_once(e);
+    sc_neg(b, e);   // R = s*G + (n - e)*P
+    sc k1, k2;
+    bool n1, n2;
+    glv_split(k1, k2, n1, n2, b);
+    ptable T;
+    build_ptable(T, P);
+    jac RP, RG, R;
+    ecmult_var(RP, T, k1, n1, k2, n2);
+    ecmult_gen(RG, G, s);
+    jac_add(R, RG, RP);
+    if (R.inf) return false;
+    aff A;
+    jac_to_aff(A, R);
+    bool v = (A.y.v[0] & 1u) == 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v &= A.x.v[i] == rr[i];
+    return ok && v;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <int MODE>   // 0: write verdict byte, 1: collect (valid -> zero the caller's byte)
+__global__ void __launch_bounds__(kVerifyBlock)
+k_ecdsa_verify(const u8* __restrict__ msgs, const u8* __restrict__ pubs33,
+               const u8* __restrict__ sigs, u32 count, gcomb G, u8* __restrict__ out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    u32 z[8], r[8], s[8], x[8];
+    load_be32_aligned(z, msgs + 32ull * i);
+    load_be32_aligned(r, sigs + 64ull * i);
+    load_be32_aligned(s, sigs + 64ull * i + 32);
+    const u8* pk = pubs33 + 33ull * i;
+    const u32 prefix = pk[0];
+    load_be32_bytes(x, pk + 1);
+    const bool v = ecdsa_verify_one(z, r, s, prefix, x, G);
+    if (MODE == 0) {
+        out[i] = v ? 1 : 0;
+    } else if (v) {
+        out[i] = 0;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kVerifyBlock)
+k_schnorr_verify(const u8* __restrict__ msgs, const u8* __restrict__ pubx,
+                 const u8* __restrict__ sigs, u32 count, gcomb G,
+                 const u32* __restrict__ mid_g, u8* __restrict__ out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    u32 m[8], px[8], r[8], s[8], mid[8];
+    load_be_words(m, msgs + 32ull * i);
+    load_be_words(px, pubx + 32ull * i);
+    load_be_words(r, sigs + 64ull * i);
+    load_be_words(s, sigs + 64ull * i + 32);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mid[j] = mid_g[j];
+    const bool v = schnorr_verify_one(m, px, r, s, mid, G);
+    if (MODE == 0) {
+        out[i] = v ? 1 : 0;
+    } else if (v) {
+        out[i] = 0;
+    }
+}
+
+// entry(i, j) = (2j+1) * 2^(w*i) * G for i < nwin; entry(nwin, 0) = 2^(w*nwin) G.
+__global__ void __launch_bounds__(64) k_build_gcomb(aff* tab, int w, int nwin) {
+    const int half = 1 << (w - 1);
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = nwin * half + 1;
+    if (id >= total) return;
+    const int i = id / half, j = id % half;
+    aff g;
+    set_generator(g);
+    jac B;
+    jac_from_aff(B, g);
+    for (int d = 0; d < w * i; ++d) jac_dbl(B, B);
+    const u32 m = (i == nwin) ? 1u : (u32)(2 * j + 1);
+    jac acc = B;
+    int top = 31 - __builtin_clz(m);
+    for (int b = top - 1; b >= 0; --b) {
+        jac_dbl(acc, acc);
+        if ((m >> b) & 1u) jac_add(acc, acc, B);
+    }
+    aff a;
+    jac_to_aff(a, acc);
+    tab[id] = a;
+}
+
+// Midstate of SHA256 after absorbing SHA256(tag) || SHA256(tag) for
+// tag = "BIP0340/challenge" (BIP-340 tagged hash).
+__global__ void k_bip340_midstate(u32* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const u32 iv[8] = {UFS_SHA256_IV};
+    u32 st[8], blk[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st[i] = iv[i];
+    // "BIP0340/challenge" = 17 bytes, single padded block
+    blk[0] = 0x42495030U;  // BIP0
+    blk[1] = 0x3334302FU;  // 340/
+    blk[2] = 0x6368616CU;  // chal
+    blk[3] = 0x6C656E67U;  // leng
+    blk[4] = 0x65800000U;  // e + pad
+    for (int i = 5; i < 15; ++i) blk[i] = 0;
+    blk[15] = 17 * 8;
+    sha256_compress(st, blk);
+    u32 blk2[16];
+    for (int i = 0; i < 8; ++i) { blk2[i] = st[i]; blk2[8 + i] = st[i]; }
+    u32 mid[8];
+    for (int i = 0; i < 8; ++i) mid[i] = iv[i];
+    sha256_compress(mid, blk2);
+    for (int i = 0; i < 8; ++i) out[i] = mid[i];
+}
+
+// Arithmetic self-test kernel: op codes documented in kernels.h.
+__global__ void __launch_bounds__(64)
+k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __restrict__ out,
+           u32 count, gcomb G) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    fe x, y, r;
+    sc sx, sy, sr;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        x.v[l] = a[16 * i + l];
+        y.v[l] = b[16 * i + l];
+        sx.v[l] = x.v[l];
+        sy.v[l] = y.v[l];
+    }
+    u32* o = out + 32 * i;
+    for (int l = 0; l < 32; ++l) o[l] = 0;
+    switch (op) {
+    case kOpFeMul: fe_mul(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeSqr: fe_sqr(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeAdd: fe_add(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeSub: fe_sub(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeInv: fe_inv(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeSqrt: { bool ok = fe_sqrt(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; o[8] = ok; } break;
+    case kOpScMul: sc_mul(sr, sx, sy); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
+    case kOpScInv: sc_inv(sr, sx); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
+    case kOpGlv: {
+        sc k1, k2; bool n1, n2;
+        glv_split(k1, k2, n1, n2, sx);
+        for (int l = 0; l < 8; ++l) { o[l] = k1.v[l]; o[8 + l] = k2.v[l]; }
+        o[16] = n1; o[17] = n2;
+    } break;
+    case kOpGenMul: {   // a*G -> affine (x, y), o[16] = infinity flag
+        jac R;
+        ecmult_gen(R, G, sx);
+        o[16] = R.inf;
+        if (!R.inf) { aff A; jac_to_aff(A, R); for (int l = 0; l < 8; ++l) { o[l] = A.x.v[l]; o[8 + l] = A.y.v[l]; } }
+    } break;
+    case kOpVarMul: {   // b*P where P = lift_x(a, even y) -> affine
+        aff P;
+        u32 xr[8];
+        for (int l = 0; l < 8; ++l) xr[l] = a[16 * i + l];
+        bool ok = lift_x(P, xr, 0);
+        sc k1, k2; bool n1, n2;
+        glv_split(k1, k2, n1, n2, sy);
+        ptable T;
+        build_ptable(T, P);
+        jac R;
+        ecmult_var(R, T, k1, n1, k2, n2);
+        o[16] = R.inf; o[17] = ok;
+        if (!R.inf) { aff A; jac_to_aff(A, R); for (int l = 0; l < 8; ++l) { o[l] = A.x.v[l]; o[8 + l] = A.y.v[l]; } }
+    } break;
+    default: break;
+    }
+}
+
+}  // namespace ufs
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (plain C++ linkage inside the library)
+// ---------------------------------------------------------------------------
+namespace ufs {
+
+static inline unsigned grid_for(u32 count, unsigned block) { return (count + block - 1) / block; }
+
+hipError_t launch_ecdsa_verify(const u8* msgs, const u8* pubs33, const u8* sigs, u32 count,
+                               const GcombDesc& g, u8* out, bool collect, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
+    if (collect)
+        hipLaunchKernelGGL(k_ecdsa_verify<1>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
+                           msgs, pubs33, sigs, count, G, out);
+    else
+        hipLaunchKernelGGL(k_ecdsa_verify<0>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
+                           msgs, pubs33, sigs, count, G, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_schnorr_verify(const u8* msgs, const u8* pubx, const u8* sigs, u32 count,
+                                 const GcombDesc& g, const u32* mid, u8* out, bool collect, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
+    if (collect)
+        hipLaunchKernelGGL(k_schnorr_verify<1>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
+                           msgs, pubx, sigs, count, G, mid, out);
+    else
+        hipLaunchKernelGGL(k_schnorr_verify<0>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
+                           msgs, pubx, sigs, count, G, mid, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st) {
+    const int total = nwin * (1 << (w - 1)) + 1;
+    hipLaunchKernelGGL(k_build_gcomb, dim3((total + 63) / 64), dim3(64), 0, st,
+                       reinterpret_cast<aff*>(table), w, nwin);
+    return hipGetLastError();
+}
+
+hipError_t launch_bip340_midstate(u32* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_bip340_midstate, dim3(1), dim3(64), 0, st, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest(int op, const u32* a, const u32* b, u32* out, u32 count,
+                           const GcombDesc& g, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
+    hipLaunchKernelGGL(k_selftest, dim3(grid_for(count, 64)), dim3(64), 0, st, op, a, b, out, count, G);
+    return hipGetLastError();
+}
+
+}  // namespace ufs

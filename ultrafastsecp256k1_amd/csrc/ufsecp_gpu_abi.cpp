@@ -1,0 +1,292 @@
+// extern "C" ufsecp GPU ABI over RocmBackend.
+//
+// Argument checking, output clearing and exception mapping follow the
+// reference C ABI shim src/cpu/src/ufsecp_gpu_impl.cpp:76-140 (helpers),
+// 182-338 (lifecycle + ECDSA), 378-399 (Schnorr), 453-493 (collect),
+// 541-558 (MSM), 1136-1158 (error strings).
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+
+#include "../../include/ufsecp/ufsecp_gpu.h"
+#include "../../include/ufsecp/ufsecp_gpu_ext.h"
+#include "backend.hpp"
+#include "msm.hpp"
+
+using ufs::GpuError;
+using ufs::RocmBackend;
+
+namespace {
+
+constexpr size_t kMaxGpuBatchN = size_t{1} << 26;
+
+ufsecp_error_t to_abi(GpuError e) { return static_cast<ufsecp_error_t>(static_cast<int>(e)); }
+
+bool clear_out(void* out, size_t count, size_t stride) {
+    if (!out || count == 0 || stride == 0) return true;
+    if (count > std::numeric_limits<size_t>::max() / stride) return false;
+    std::memset(out, 0, count * stride);
+    return true;
+}
+
+ufsecp_error_t to_abi_clear(GpuError e, void* out, size_t count, size_t stride) {
+    const ufsecp_error_t r = to_abi(e);
+    if (r != UFSECP_OK) (void)clear_out(out, count, stride);
+    return r;
+}
+
+}  // namespace
+
+struct ufsecp_gpu_ctx {
+    std::unique_ptr<RocmBackend> backend;
+    std::unique_ptr<ufs::MsmEngine> msm;
+    uint32_t backend_id;
+    uint32_t device_index;
+};
+
+#define UFSECP_GPU_CATCH                                           \
+    catch (const std::bad_alloc&) { return UFSECP_ERR_GPU_MEMORY; } \
+    catch (...) { return UFSECP_ERR_INTERNAL; }
+
+extern "C" {
+
+uint32_t ufsecp_gpu_backend_count(uint32_t* backend_ids, uint32_t max_ids) noexcept {
+    if (backend_ids && max_ids > 0) backend_ids[0] = UFSECP_GPU_BACKEND_ROCM;
+    return 1;
+}
+
+const char* ufsecp_gpu_backend_name(uint32_t backend_id) {
+    switch (backend_id) {
+    case UFSECP_GPU_BACKEND_CUDA: return "CUDA";
+    case UFSECP_GPU_BACKEND_OPENCL: return "OpenCL";
+    case UFSECP_GPU_BACKEND_METAL: return "Metal";
+    case UFSECP_GPU_BACKEND_ROCM: return "ROCm";
+    default: return "none";
+    }
+}
+
+int ufsecp_gpu_is_available(uint32_t backend_id) noexcept {
+    if (backend_id != UFSECP_GPU_BACKEND_ROCM) return 0;
+    return RocmBackend::device_count() > 0 ? 1 : 0;
+}
+
+uint32_t ufsecp_gpu_device_count(uint32_t backend_id) noexcept {
+    if (backend_id != UFSECP_GPU_BACKEND_ROCM) return 0;
+    const int n = RocmBackend::device_count();
+    return n > 0 ? (uint32_t)n : 0u;
+}
+
+ufsecp_error_t ufsecp_gpu_device_info(uint32_t backend_id, uint32_t device_index,
+                                      ufsecp_gpu_device_info_t* info_out) {
+    if (!info_out) return UFSECP_ERR_NULL_ARG;
+    if (backend_id != UFSECP_GPU_BACKEND_ROCM) return UFSECP_ERR_GPU_UNAVAILABLE;
+    try {
+        ufs::DeviceInfoData d;
+        const GpuError e = RocmBackend::device_info(device_index, d);
+        if (e != GpuError::Ok) return to_abi(e);
+        std::memcpy(info_out->name, d.name, sizeof(info_out->name));
+        info_out->global_mem_bytes = d.global_mem_bytes;
+        info_out->compute_units = d.compute_units;
+        info_out->max_clock_mhz = d.max_clock_mhz;
+        info_out->max_threads_per_block = d.max_threads_per_block;
+        info_out->backend_id = backend_id;
+        info_out->device_index = device_index;
+        return UFSECP_OK;
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ctx_create(ufsecp_gpu_ctx** ctx_out, uint32_t backend_id, uint32_t device_index) {
+    if (!ctx_out) return UFSECP_ERR_NULL_ARG;
+    *ctx_out = nullptr;
+    try {
+        if (backend_id != UFSECP_GPU_BACKEND_ROCM) return UFSECP_ERR_GPU_UNAVAILABLE;
+        auto backend = std::make_unique<RocmBackend>();
+        const GpuError e = backend->init(device_index);
+        if (e != GpuError::Ok) return to_abi(e);
+        auto* ctx = new (std::nothrow) ufsecp_gpu_ctx;
+        if (!ctx) return UFSECP_ERR_INTERNAL;
+        ctx->msm = std::make_unique<ufs::MsmEngine>(backend.get());
+        ctx->backend = std::move(backend);
+        ctx->backend_id = backend_id;
+        ctx->device_index = device_index;
+        *ctx_out = ctx;
+        return UFSECP_OK;
+    }
+    UFSECP_GPU_CATCH
+}
+
+void ufsecp_gpu_ctx_destroy(ufsecp_gpu_ctx* ctx) {
+    if (!ctx) return;
+    ctx->msm.reset();
+    ctx->backend.reset();
+    delete ctx;
+}
+
+int ufsecp_gpu_is_ready(const ufsecp_gpu_ctx* ctx) noexcept {
+    if (!ctx || !ctx->backend) return 0;
+    return ctx->backend->is_ready() ? 1 : 0;
+}
+
+ufsecp_error_t ufsecp_gpu_last_error(const ufsecp_gpu_ctx* ctx) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    return to_abi(ctx->backend->last_error());
+}
+
+const char* ufsecp_gpu_last_error_msg(const ufsecp_gpu_ctx* ctx) {
+    if (!ctx) return "NULL GPU context";
+    return ctx->backend->last_error_msg();
+}
+
+const char* ufsecp_gpu_error_str(ufsecp_error_t err) {
+    switch (err) {
+    case UFSECP_OK: return "OK";
+    case UFSECP_ERR_NULL_ARG: return "NULL argument";
+    case UFSECP_ERR_BAD_KEY: return "invalid private key";
+    case UFSECP_ERR_BAD_PUBKEY: return "invalid public key";
+    case UFSECP_ERR_BAD_SIG: return "invalid signature";
+    case UFSECP_ERR_BAD_INPUT: return "malformed input";
+    case UFSECP_ERR_VERIFY_FAIL: return "verification failed";
+    case UFSECP_ERR_ARITH: return "arithmetic error";
+    case UFSECP_ERR_SELFTEST: return "self-test failed";
+    case UFSECP_ERR_INTERNAL: return "internal error";
+    case UFSECP_ERR_BUF_TOO_SMALL: return "buffer too small";
+    case UFSECP_ERR_GPU_UNAVAILABLE: return "GPU backend unavailable";
+    case UFSECP_ERR_GPU_DEVICE: return "GPU device error";
+    case UFSECP_ERR_GPU_LAUNCH: return "GPU kernel launch failed";
+    case UFSECP_ERR_GPU_MEMORY: return "GPU memory error";
+    case UFSECP_ERR_GPU_UNSUPPORTED: return "operation not supported on this GPU backend";
+    case UFSECP_ERR_GPU_BACKEND: return "GPU backend driver error";
+    case UFSECP_ERR_GPU_QUEUE: return "GPU command queue error";
+    default: return "unknown error";
+    }
+}
+
+ufsecp_error_t ufsecp_gpu_ecdsa_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                             const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
+                                             uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_results, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !pubkeys33 || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(
+            ctx->backend->ecdsa_verify_batch(msg_hashes32, pubkeys33, sigs64, count, out_results, false),
+            out_results, count, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_schnorr_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                               const uint8_t* pubkeys_x32, const uint8_t* sigs64, size_t count,
+                                               uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_results, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !pubkeys_x32 || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(
+            ctx->backend->schnorr_verify_batch(msg_hashes32, pubkeys_x32, sigs64, count, out_results, false),
+            out_results, count, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ecdsa_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                               const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
+                                               uint8_t* key_buffer) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (!msg_hashes32 || !pubkeys33 || !sigs64 || !key_buffer) return UFSECP_ERR_NULL_ARG;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    try {
+        return to_abi(ctx->backend->ecdsa_verify_batch(msg_hashes32, pubkeys33, sigs64, count, key_buffer, true));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_schnorr_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                 const uint8_t* pubkeys_x32, const uint8_t* sigs64, size_t count,
+                                                 uint8_t* key_buffer) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (!msg_hashes32 || !pubkeys_x32 || !sigs64 || !key_buffer) return UFSECP_ERR_NULL_ARG;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    try {
+        return to_abi(
+            ctx->backend->schnorr_verify_batch(msg_hashes32, pubkeys_x32, sigs64, count, key_buffer, true));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_msm(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32, const uint8_t* points33, size_t n,
+                              uint8_t* out_result33) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_result33, 1, 33)) return UFSECP_ERR_BAD_INPUT;
+    if (!scalars32 || !points33 || !out_result33) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->msm->msm_host(scalars32, points33, n, out_result33, 0), out_result33, 1, 33);
+    }
+    UFSECP_GPU_CATCH
+}
+
+// ---- extensions ------------------------------------------------------------
+
+ufsecp_error_t ufsecp_gpu_ext_ecdsa_verify_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_msg_hashes32,
+                                                  const uint8_t* d_pubkeys33, const uint8_t* d_sigs64, size_t count,
+                                                  uint8_t* d_out_results, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!d_msg_hashes32 || !d_pubkeys33 || !d_sigs64 || !d_out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->backend->ecdsa_verify_device(d_msg_hashes32, d_pubkeys33, d_sigs64, count, d_out_results,
+                                                        (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_schnorr_verify_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_msg_hashes32,
+                                                    const uint8_t* d_pubkeys_x32, const uint8_t* d_sigs64,
+                                                    size_t count, uint8_t* d_out_results, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!d_msg_hashes32 || !d_pubkeys_x32 || !d_sigs64 || !d_out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->backend->schnorr_verify_device(d_msg_hashes32, d_pubkeys_x32, d_sigs64, count,
+                                                          d_out_results, (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+void* ufsecp_gpu_ext_stream(ufsecp_gpu_ctx* ctx) {
+    if (!ctx) return nullptr;
+    return (void*)ctx->backend->stream();
+}
+
+ufsecp_error_t ufsecp_gpu_ext_synchronize(ufsecp_gpu_ctx* ctx) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->backend->synchronize());
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_selftest(ufsecp_gpu_ctx* ctx, int op, const uint32_t* a, const uint32_t* b,
+                                       uint32_t* out, size_t count) {
+    if (!ctx || !a || !b || !out) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > (size_t{1} << 20)) return UFSECP_ERR_BAD_INPUT;
+    try {
+        return to_abi(ctx->backend->selftest(op, a, b, out, count));
+    }
+    UFSECP_GPU_CATCH
+}
+
+}  // extern "C"

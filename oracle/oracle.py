@@ -1,0 +1,97 @@
+"""ctypes view of the C oracle (oracle/liboracle.so) -- test infrastructure only.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never
+by the product library.  See secp256k1_oracle.c for the reference functions
+each entry point restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+_LIB = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.is_file():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        L.orc_ecdsa_verify.argtypes = [vp, vp, vp]
+        L.orc_schnorr_verify.argtypes = [vp, vp, vp]
+        L.orc_msm.argtypes = [vp, vp, sz, vp]
+        L.orc_pubkey33.argtypes = [vp, vp]
+        L.orc_ecdsa_sign.argtypes = [vp, vp, vp, vp]
+        L.orc_schnorr_sign.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_verify_batch.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp, ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(b):
+    if isinstance(b, np.ndarray):
+        return ctypes.c_void_p(b.ctypes.data)
+    if b is None:
+        return None
+    return ctypes.cast(ctypes.c_char_p(bytes(b)), ctypes.c_void_p)
+
+
+def ecdsa_verify(msg: bytes, sig: bytes, pub33: bytes) -> int:
+    return lib().orc_ecdsa_verify(msg, sig, pub33)
+
+
+def schnorr_verify(msg: bytes, sig: bytes, px: bytes) -> int:
+    return lib().orc_schnorr_verify(msg, sig, px)
+
+
+def msm(scalars: bytes, points: bytes, n: int):
+    out = ctypes.create_string_buffer(33)
+    rc = lib().orc_msm(bytes(scalars), bytes(points), n, out)
+    return rc, out.raw
+
+
+def pubkey33(d: bytes) -> bytes:
+    out = ctypes.create_string_buffer(33)
+    assert lib().orc_pubkey33(d, out) == 1
+    return out.raw
+
+
+def ecdsa_sign(msg: bytes, d: bytes):
+    sig = ctypes.create_string_buffer(64)
+    k = ctypes.create_string_buffer(32)
+    assert lib().orc_ecdsa_sign(msg, d, sig, k) == 1
+    return sig.raw, k.raw
+
+
+def schnorr_sign(msg: bytes, d: bytes, aux: bytes):
+    sig = ctypes.create_string_buffer(64)
+    px = ctypes.create_string_buffer(32)
+    assert lib().orc_schnorr_sign(msg, d, aux, sig, px) == 1
+    return sig.raw, px.raw
+
+
+def verify_batch(kind: str, msgs: np.ndarray, pubs: np.ndarray, sigs: np.ndarray, threads: int = 0) -> np.ndarray:
+    n = msgs.shape[0]
+    out = np.zeros(n, np.uint8)
+    msgs, pubs, sigs = (np.ascontiguousarray(a) for a in (msgs, pubs, sigs))
+    threads = threads or min(16, os.cpu_count() or 1)
+    lib().orc_verify_batch(1 if kind == "schnorr" else 0, _p(msgs), _p(pubs), _p(sigs), n, _p(out), threads)
+    return out
+
+
+def verify_range(kind: str, msgs, pubs, sigs) -> int:
+    """Single-threaded verification of a slice; returns the number valid."""
+    return int(verify_batch(kind, msgs, pubs, sigs, threads=1).sum())

@@ -193,19 +193,28 @@ def bip340_vectors(ref: Ref) -> list:
 
 def rfc6979_vectors(ref: Ref) -> list:
     src = (REF_SRC / "src/cpu/tests/test_rfc6979_vectors.cpp").read_text()
+    nonce_sec = src[src.index("static void test_rfc6979_nonce_vectors"):src.index("static void test_ecdsa_sign_vectors")]
+    sig_sec = src[src.index("static void test_ecdsa_sign_vectors"):src.index("static void test_ecdsa_verify_roundtrip")]
     cases = []
-    for blk in re.split(r"// Vector \d+:", src)[1:]:
-        ds = re.findall(r'"([0-9a-fA-F]{64})"', blk)
-        mm = re.search(r'sha256_str\(\s*((?:"[^"]*"\s*)+)\)', blk)
-        if len(ds) < 2 or not mm:
-            continue
-        text = "".join(re.findall(r'"([^"]*)"', mm.group(1)))
-        d, k = ds[0], ds[1]
-        mh = hashlib.sha256(text.encode()).digest()
-        sig = ref.ecdsa_sign(mh, bytes.fromhex(d))
-        cases.append({"d": d, "message": text, "msg_hash": mh.hex(), "k": k, "sig": sig.hex(),
-                      "pub": ref.pubkey33(bytes.fromhex(d)).hex()})
-    assert len(cases) >= 6, len(cases)
+    for section, kind in ((nonce_sec, "nonce"), (sig_sec, "signature")):
+        for blk in re.split(r"// Vector \d+:", section)[1:]:
+            ds = re.findall(r'"([0-9a-fA-F]{64})"', blk)
+            mm = re.search(r'sha256_str\(\s*((?:"[^"]*"\s*)+)\)', blk)
+            if not ds or not mm:
+                continue
+            text = "".join(re.findall(r'"([^"]*)"', mm.group(1)))
+            mh = hashlib.sha256(text.encode()).digest()
+            d = ds[0]
+            sig = ref.ecdsa_sign(mh, bytes.fromhex(d))
+            c = {"kind": kind, "d": d, "message": text, "msg_hash": mh.hex(), "sig": sig.hex(),
+                 "pub": ref.pubkey33(bytes.fromhex(d)).hex()}
+            if kind == "nonce":
+                c["k"] = ds[1].lower()
+            else:
+                c["r"], c["s"] = ds[1].lower(), ds[2].lower()
+                assert sig.hex() == c["r"] + c["s"], ("reference signer vs its own vector", text)
+            cases.append(c)
+    assert sum(c["kind"] == "nonce" for c in cases) >= 6 and sum(c["kind"] == "signature" for c in cases) >= 7
     return cases
 
 

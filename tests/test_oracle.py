@@ -1,0 +1,97 @@
+"""Pin the C oracle (oracle/secp256k1_oracle.c) before trusting it.
+
+Checks against data recorded from the reference CPU library and the
+reference's own vectors (tests/golden/*.json), and against closed forms.  A
+planted mistake in the oracle (a dropped reduction, a wrong sign, a swapped
+operand) breaks one of these.
+"""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+from ec_math import G, N, P, compress, lift_x, mul
+from oracle import oracle as orc
+from ultrafastsecp256k1_amd import workload as wl
+
+
+def test_bip340_vectors(golden):
+    for c in golden["bip340_vectors"]:
+        v = orc.schnorr_verify(bytes.fromhex(c["msg"]), bytes.fromhex(c["sig"]), bytes.fromhex(c["pubkey"]))
+        assert v == c["verdict"], c["vector"]
+        if "sk" in c:
+            sig, px = orc.schnorr_sign(bytes.fromhex(c["msg"]), bytes.fromhex(c["sk"]), bytes.fromhex(c["aux"]))
+            assert sig.hex().upper() == c["sig"].upper()
+            assert px.hex().upper() == c["pubkey"].upper()
+
+
+def test_rfc6979_vectors(golden):
+    for c in golden["rfc6979_vectors"]:
+        sig, k = orc.ecdsa_sign(bytes.fromhex(c["msg_hash"]), bytes.fromhex(c["d"]))
+        if c["kind"] == "nonce":
+            assert k.hex() == c["k"], c["message"]
+        else:
+            assert sig.hex() == c["r"] + c["s"], c["message"]
+        assert sig.hex() == c["sig"]
+        assert orc.pubkey33(bytes.fromhex(c["d"])).hex() == c["pub"]
+        assert orc.ecdsa_verify(bytes.fromhex(c["msg_hash"]), sig, bytes.fromhex(c["pub"])) == 1
+
+
+@pytest.mark.parametrize("kind", ["ecdsa", "schnorr"])
+def test_edge_cases(golden, kind):
+    fn = orc.ecdsa_verify if kind == "ecdsa" else orc.schnorr_verify
+    for c in golden["edge_cases"][kind]:
+        assert fn(bytes.fromhex(c["msg"]), bytes.fromhex(c["sig"]), bytes.fromhex(c["pub"])) == c["verdict"], c["name"]
+
+
+def test_msm_cases(golden):
+    for c in golden["edge_cases"]["msm"]:
+        rc, out = orc.msm(bytes.fromhex(c["scalars"]), bytes.fromhex(c["points"]), c["n"])
+        assert rc == c["rc"], c["name"]
+        if rc == 0:
+            assert out.hex() == c["result"], c["name"]
+
+
+@pytest.mark.parametrize("kind,n", [("ecdsa", 4096), ("schnorr", 8192)])
+def test_workload_verdicts_match_reference(golden, kind, n):
+    f = golden["workload_heads"][kind]
+    b = wl.generate(kind, f["n"], f["seed"])
+    assert hashlib.sha256(b.sigs.tobytes()).hexdigest() == f["sha256_sigs"]
+    v = orc.verify_batch(kind, b.msgs[:n], b.pubkeys[:n], b.sigs[:n])
+    if n == f["n"]:
+        assert hashlib.sha256(v.tobytes()).hexdigest() == f["sha256_verdicts"]
+    head = [h["verdict"] for h in f["head"]]
+    assert list(v[:len(head)]) == head
+    corr = b.corrupted[:n].astype(bool)
+    assert np.all(v[~corr] == 1) and np.all(v[corr] == 0)
+
+
+def test_signers_match_reference_head(golden):
+    for kind in ("ecdsa", "schnorr"):
+        f = golden["workload_heads"][kind]
+        for i, h in enumerate(f["head"][:8]):
+            spec = (wl.ecdsa_item_spec if kind == "ecdsa" else wl.schnorr_item_spec)(f["seed"], i)
+            if kind == "ecdsa":
+                sig, _ = orc.ecdsa_sign(spec["msg"], spec["d"])
+                pub = orc.pubkey33(spec["d"])
+            else:
+                sig, pub = orc.schnorr_sign(spec["msg"], spec["d"], spec["aux"])
+            msg, sig = wl.apply_corruption(spec, spec["msg"], sig)
+            assert (msg.hex(), pub.hex(), sig.hex()) == (h["msg"], h["pub"], h["sig"])
+
+
+def test_msm_closed_form():
+    rnd = random.Random(11)
+    ks = [rnd.randrange(1, N) for _ in range(24)]
+    ss = [rnd.randrange(N) for _ in range(24)]
+    pts = b"".join(compress(mul(k)) for k in ks)
+    rc, out = orc.msm(b"".join(s.to_bytes(32, "big") for s in ss), pts, 24)
+    assert rc == 0
+    assert out == compress(mul(sum(s * k for s, k in zip(ss, ks)) % N))
+
+
+def test_textbook_helpers_consistent():
+    # the Python helpers used to pin device primitives agree with closed forms
+    assert mul(N - 1) == (G[0], P - G[1])
+    assert lift_x(G[0], odd=bool(G[1] & 1)) == G

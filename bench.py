@@ -51,52 +51,29 @@ def measure_valu_peak() -> dict:
     return out
 
 
-def cpu_baseline(kind: str, batch, cores: int, budget_items: int) -> dict:
-    """Per-item CPU verification of a bounded sample on the host cores."""
-    ref = ROOT / "oracle" / "_ref" / "libufsecp_ref.so"
+def cpu_baseline(kind: str, batch, cores: int, budget_items: int, gpu_verdicts=None) -> dict:
+    """Per-item CPU verification of a bounded sample on the host cores: the
+    reference CPU library (oracle/_ref, built from the reference sources) when
+    present, else the C oracle restatement.  C threads, one context each."""
+    from oracle import oracle as orc
     n = min(budget_items, batch.n)
-    if ref.is_file():
-        os.environ.setdefault("SECP256K1_CACHE_DIR", "/tmp/ufsecp_ref_cache")
-        os.makedirs(os.environ["SECP256K1_CACHE_DIR"], exist_ok=True)
-        lib = ctypes.CDLL(str(ref))
+    m, p, s = batch.msgs[:n], batch.pubkeys[:n], batch.sigs[:n]
+    if orc.ref_available():
         kind_cpu = "reference"
-        tls = threading.local()
-
-        def ctx():
-            c = getattr(tls, "c", None)
-            if c is None:
-                c = ctypes.c_void_p()
-                assert lib.ufsecp_ctx_create(ctypes.byref(c)) == 0
-                tls.c = c
-            return c
-
-        fn = lib.ufsecp_schnorr_verify if kind == "schnorr" else lib.ufsecp_ecdsa_verify
-
-        def verify_range(lo, hi):
-            c = ctx()
-            m, p, s = batch.msgs, batch.pubkeys, batch.sigs
-            ok = 0
-            for i in range(lo, hi):
-                ok += fn(c, m[i].ctypes.data, s[i].ctypes.data, p[i].ctypes.data) == 0
-            return ok
+        orc.ref_verify_batch(kind, m[:cores], p[:cores], s[:cores], cores)   # warm-up: tables, caches
+        verdicts, dt = orc.ref_verify_batch(kind, m, p, s, cores)
     else:
-        from oracle import oracle as orc  # restatement of the reference verifier
         kind_cpu = "port"
-
-        def verify_range(lo, hi):
-            return orc.verify_range(kind, batch.msgs[lo:hi], batch.pubkeys[lo:hi], batch.sigs[lo:hi])
-
-    # warm-up (pubkey caches / table build), then timed
-    with ThreadPoolExecutor(cores) as ex:
-        list(ex.map(lambda t: verify_range(t, t + 1), range(cores)))
-    chunk = (n + cores - 1) // cores
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(cores) as ex:
-        oks = list(ex.map(lambda t: verify_range(t * chunk, min(n, (t + 1) * chunk)), range(cores)))
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "sigs/s", "cores": cores, "kind": kind_cpu,
-            "sample": f"first {n} items of the same {kind} workload, per-item verify, {cores} threads",
-            "valid_in_sample": int(sum(oks)), "seconds": round(dt, 3)}
+        t0 = time.perf_counter()
+        verdicts = orc.verify_batch(kind, m, p, s, threads=cores)
+        dt = time.perf_counter() - t0
+    out = {"value": n / dt, "unit": "sigs/s", "cores": cores, "kind": kind_cpu,
+           "sample": f"first {n} items of the same {kind} workload, per-item verify, {cores} threads",
+           "seconds": round(dt, 3), "valid_in_sample": int(verdicts.sum())}
+    if gpu_verdicts is not None:
+        import numpy as np
+        out["matches_gpu"] = bool(np.array_equal(verdicts, gpu_verdicts[:n]))
+    return out
 
 
 def main() -> int:
@@ -106,7 +83,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="schnorr", choices=["schnorr", "ecdsa"])
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--cpu-items", type=int, default=1 << 17)
+    ap.add_argument("--cpu-items", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -115,7 +92,6 @@ def main() -> int:
         return 2
 
     import numpy as np
-    import torch
     from ultrafastsecp256k1_amd import UfsecpGpu
     from ultrafastsecp256k1_amd import workload as wl
 
@@ -123,32 +99,28 @@ def main() -> int:
     t0 = time.perf_counter()
     batch = wl.generate(kind, n, threads=16)
     log(f"[bench] generated {n} {kind} items in {time.perf_counter() - t0:.1f}s")
-    dev = torch.device("cuda:0")
-    d_m = torch.from_numpy(batch.msgs).to(dev)
-    d_p = torch.from_numpy(batch.pubkeys).to(dev)
-    d_s = torch.from_numpy(batch.sigs).to(dev)
-    d_o = torch.zeros(n, dtype=torch.uint8, device=dev)
     gpu = UfsecpGpu()
-    stream = torch.cuda.current_stream()
+    d_m, d_p, d_s = (gpu.to_device(a) for a in (batch.msgs, batch.pubkeys, batch.sigs))
+    d_o = gpu.empty(n)
     launch = gpu.schnorr_verify_device if kind == "schnorr" else gpu.ecdsa_verify_device
 
+    # All launches and events go to the context's HIP stream.
     def step():
-        launch(d_m.data_ptr(), d_p.data_ptr(), d_s.data_ptr(), n, d_o.data_ptr(), stream.cuda_stream)
+        launch(d_m.ptr, d_p.ptr, d_s.ptr, n, d_o.ptr, None)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    gpu.synchronize()
+    evs = [(gpu.event(), gpu.event()) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for a, b in evs:
-        a.record(stream)
+        a.record()
         step()
-        b.record(stream)
-    torch.cuda.synchronize()
+        b.record()
+    gpu.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in evs]
-    verdicts = d_o.cpu().numpy()
+    kern_ms = [a.elapsed_ms(b) for a, b in evs]
+    verdicts = d_o.download()
     # parity: every untouched signature verifies, every corrupted one fails
     corr = batch.corrupted.astype(bool)
     parity_ok = bool(np.all(verdicts[~corr] == 1) and np.all(verdicts[corr] == 0))
@@ -171,7 +143,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(kind, batch, cores, args.cpu_items)
+        cpu = cpu_baseline(kind, batch, cores, args.cpu_items, verdicts)
         log(f"[bench] cpu baseline: {cpu}")
     out = {
         "metric": METRIC, "value": value, "unit": "sigs/s", "n_gpus": 1, "steps": args.steps,

@@ -30,6 +30,62 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_schnorr_verify_device(ufsecp_gpu_ctx* c
                                                                const uint8_t* d_pubkeys_x32, const uint8_t* d_sigs64,
                                                                size_t count, uint8_t* d_out_results, void* stream);
 
+/* One batch of a multi-batch verification (all pointers are device
+ * addresses; strides as in the single-batch calls: msgs 32 B, pubkeys 33 B for
+ * ECDSA or 32 B for BIP-340, sigs 64 B, out 1 B per item). */
+#define UFSECP_GPU_KIND_ECDSA   0u
+#define UFSECP_GPU_KIND_SCHNORR 1u
+typedef struct {
+    uint32_t kind;
+    uint32_t count;
+    const uint8_t* msg_hashes32;
+    const uint8_t* pubkeys;
+    const uint8_t* sigs64;
+    uint8_t* out_results;
+} ufsecp_gpu_verify_batch_t;
+
+/* Verify `nbatches` batches of uneven sizes and mixed kinds in ONE persistent
+ * launch: every wave pulls 64-item tiles from a device work queue.  With
+ * use_graph != 0 the launch is replayed from a captured hipGraph (captured on
+ * first use, re-captured only when internal tables are reallocated).
+ * Asynchronous on `stream` (NULL = context stream).  Verdicts are identical to
+ * the per-batch calls. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_verify_multi_device(ufsecp_gpu_ctx* ctx,
+                                                             const ufsecp_gpu_verify_batch_t* batches,
+                                                             size_t nbatches, int use_graph, void* stream);
+
+/* The plain form of the above: one kernel launch per batch (for comparison). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_verify_each_device(ufsecp_gpu_ctx* ctx,
+                                                            const ufsecp_gpu_verify_batch_t* batches,
+                                                            size_t nbatches, void* stream);
+
+/* ufsecp_gpu_msm with an explicit Pippenger window width c (2..20; 0 = the
+ * size-based default).  Same inputs, outputs and error codes. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32, const uint8_t* points33,
+                                             size_t n, uint8_t* out_result33, int window_bits);
+
+/* Device-pointer MSM, asynchronous on `stream`: writes the 33-byte result to
+ * d_out33 and the ABI status code (0, BAD_INPUT, BAD_PUBKEY or ERR_ARITH) to
+ * *d_status, both device addresses. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
+                                                    const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                                                    int32_t* d_status, int window_bits, void* stream);
+
+/* Device memory and timing helpers for callers without their own HIP runtime
+ * handle (tests, benchmarks, language bindings).  Copies are synchronous with
+ * respect to the context stream. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** d_ptr_out);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_free(ufsecp_gpu_ctx* ctx, void* d_ptr);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_copy_to_device(ufsecp_gpu_ctx* ctx, void* d_dst, const void* h_src,
+                                                        size_t bytes);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_copy_to_host(ufsecp_gpu_ctx* ctx, void* h_dst, const void* d_src,
+                                                      size_t bytes);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_event_create(ufsecp_gpu_ctx* ctx, void** event_out);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_event_record(ufsecp_gpu_ctx* ctx, void* event, void* stream);
+/* Milliseconds between two recorded events (waits for `stop`). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_event_elapsed(void* start, void* stop, float* ms_out);
+UFSECP_API void ufsecp_gpu_ext_event_destroy(void* event);
+
 /* The context's HIP stream (hipStream_t) for callers that want to order their
  * own work after the library's. */
 UFSECP_API void* ufsecp_gpu_ext_stream(ufsecp_gpu_ctx* ctx);

@@ -92,6 +92,34 @@ def verify_batch(kind: str, msgs: np.ndarray, pubs: np.ndarray, sigs: np.ndarray
     return out
 
 
+REF_LIB_PATH = HERE / "_ref" / "libufsecp_ref.so"
+
+
+def ref_available() -> bool:
+    return REF_LIB_PATH.is_file()
+
+
+def ref_verify_batch(kind: str, msgs: np.ndarray, pubs: np.ndarray, sigs: np.ndarray, threads: int):
+    """Per-item verification through the reference CPU library built by
+    oracle/ref.mk (C threads, one ufsecp context each).  Returns (verdicts,
+    wall seconds)."""
+    os.environ.setdefault("SECP256K1_CACHE_DIR", "/tmp/ufsecp_ref_cache")
+    os.makedirs(os.environ["SECP256K1_CACHE_DIR"], exist_ok=True)
+    L = lib()
+    fn = L.orc_ref_verify_batch
+    fn.restype = ctypes.c_double
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int]
+    msgs, pubs, sigs = (np.ascontiguousarray(a) for a in (msgs, pubs, sigs))
+    n = msgs.shape[0]
+    out = np.zeros(n, np.uint8)
+    secs = fn(str(REF_LIB_PATH).encode(), 1 if kind == "schnorr" else 0, _p(msgs), _p(pubs), _p(sigs), n, _p(out),
+              threads)
+    if secs < 0:
+        raise RuntimeError(f"reference driver failed ({secs})")
+    return out, secs
+
+
 def verify_range(kind: str, msgs, pubs, sigs) -> int:
     """Single-threaded verification of a slice; returns the number valid."""
     return int(verify_batch(kind, msgs, pubs, sigs, threads=1).sum())

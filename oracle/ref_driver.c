@@ -1,0 +1,81 @@
+/* Multi-threaded driver for per-item verification through a CPU library that
+ * exports the reference CPU C ABI (ufsecp_ctx_create, ufsecp_ecdsa_verify,
+ * ufsecp_schnorr_verify) -- TEST INFRASTRUCTURE ONLY (CPU baseline timing and
+ * parity cross-checks).  The library is opened with dlopen so the driver has
+ * no link-time dependency on it.  One ufsecp context per thread, as the
+ * reference requires (include/ufsecp/ufsecp.h: contexts are single-thread). */
+#include <dlfcn.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <time.h>
+
+typedef int (*ctx_create_fn)(void**);
+typedef void (*ctx_destroy_fn)(void*);
+typedef int (*verify_fn)(void*, const uint8_t*, const uint8_t*, const uint8_t*);
+
+typedef struct {
+    ctx_create_fn create;
+    ctx_destroy_fn destroy;
+    verify_fn verify;
+    int schnorr;
+    const uint8_t *msgs, *pubs, *sigs;
+    uint8_t* out;
+    size_t lo, hi;
+    int rc;
+} rjob_t;
+
+static void* rjob_run(void* arg) {
+    rjob_t* j = (rjob_t*)arg;
+    void* ctx = 0;
+    if (j->create(&ctx) != 0) {
+        j->rc = -1;
+        return 0;
+    }
+    const size_t ps = j->schnorr ? 32 : 33;
+    for (size_t i = j->lo; i < j->hi; ++i)
+        j->out[i] = (uint8_t)(j->verify(ctx, j->msgs + 32 * i, j->sigs + 64 * i, j->pubs + ps * i) == 0);
+    if (j->destroy) j->destroy(ctx);
+    j->rc = 0;
+    return 0;
+}
+
+/* Verifies n items with `threads` threads; returns wall seconds (< 0 on error). */
+__attribute__((visibility("default"))) double orc_ref_verify_batch(const char* lib_path, int schnorr,
+                                                                     const uint8_t* msgs, const uint8_t* pubs,
+                                                                     const uint8_t* sigs, size_t n, uint8_t* out,
+                                                                     int threads) {
+    void* h = dlopen(lib_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1.0;
+    ctx_create_fn create = (ctx_create_fn)dlsym(h, "ufsecp_ctx_create");
+    ctx_destroy_fn destroy = (ctx_destroy_fn)dlsym(h, "ufsecp_ctx_destroy");
+    verify_fn verify = (verify_fn)dlsym(h, schnorr ? "ufsecp_schnorr_verify" : "ufsecp_ecdsa_verify");
+    if (!create || !verify) return -2.0;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    rjob_t jobs[256];
+    const size_t chunk = (n + (size_t)threads - 1) / (size_t)threads;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < threads; ++t) {
+        rjob_t* j = &jobs[t];
+        j->create = create; j->destroy = destroy; j->verify = verify; j->schnorr = schnorr;
+        j->msgs = msgs; j->pubs = pubs; j->sigs = sigs; j->out = out;
+        j->lo = (size_t)t * chunk < n ? (size_t)t * chunk : n;
+        j->hi = (size_t)(t + 1) * chunk < n ? (size_t)(t + 1) * chunk : n;
+        j->rc = 0;
+        if (pthread_create(&th[t], 0, rjob_run, j) != 0) {
+            rjob_run(j);
+            th[t] = 0;
+        }
+    }
+    int rc = 0;
+    for (int t = 0; t < threads; ++t) {
+        if (th[t]) pthread_join(th[t], 0);
+        rc |= jobs[t].rc;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (rc) return -3.0;
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

@@ -65,7 +65,8 @@ def test_edge_cases(gpu, golden, kind):
 def test_edge_cases_mixed_into_batch(gpu, golden, kind):
     """Each verdict is the same alone and inside a larger batch (lane position)."""
     cases = golden["edge_cases"][kind]
-    b = wl.generate(kind, 300, first=1000)
+    n = 64 + 13 * len(cases)
+    b = wl.generate(kind, n, first=1000)
     fn = gpu.ecdsa_verify_batch if kind == "ecdsa" else gpu.schnorr_verify_batch
     base = fn(b.msgs, b.pubkeys, b.sigs)
     msgs, pubs, sigs = b.msgs.copy(), b.pubkeys.copy(), b.sigs.copy()
@@ -77,7 +78,7 @@ def test_edge_cases_mixed_into_batch(gpu, golden, kind):
     v = fn(msgs, pubs, sigs)
     for p, c in zip(pos, cases):
         assert v[p] == c["verdict"], c["name"]
-    mask = np.ones(300, bool)
+    mask = np.ones(n, bool)
     mask[pos] = False
     assert np.array_equal(v[mask], base[mask])
 

@@ -41,7 +41,19 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_msm",
     "ufsecp_gpu_ext_ecdsa_verify_device", "ufsecp_gpu_ext_schnorr_verify_device",
     "ufsecp_gpu_ext_stream", "ufsecp_gpu_ext_synchronize", "ufsecp_gpu_ext_selftest",
+    "ufsecp_gpu_ext_verify_multi_device", "ufsecp_gpu_ext_verify_each_device",
+    "ufsecp_gpu_ext_msm", "ufsecp_gpu_ext_msm_device",
 )
+
+
+class VerifyBatchDesc(ctypes.Structure):
+    """ufsecp_gpu_verify_batch_t (include/ufsecp/ufsecp_gpu_ext.h); device pointers."""
+    _fields_ = [("kind", c_uint32), ("count", c_uint32), ("msg_hashes32", c_void_p),
+                ("pubkeys", c_void_p), ("sigs64", c_void_p), ("out_results", c_void_p)]
+
+
+KIND_ECDSA = 0
+KIND_SCHNORR = 1
 
 
 class UfsecpError(RuntimeError):
@@ -91,6 +103,18 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_ext_ecdsa_verify_device": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
         "ufsecp_gpu_ext_schnorr_verify_device": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
+        "ufsecp_gpu_ext_verify_multi_device": (c_int, [vp, vp, c_size_t, c_int, vp]),
+        "ufsecp_gpu_ext_verify_each_device": (c_int, [vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ext_msm": (c_int, [vp, vp, vp, c_size_t, vp, c_int]),
+        "ufsecp_gpu_ext_msm_device": (c_int, [vp, vp, vp, c_size_t, vp, vp, c_int, vp]),
+        "ufsecp_gpu_ext_alloc": (c_int, [vp, c_size_t, POINTER(vp)]),
+        "ufsecp_gpu_ext_free": (c_int, [vp, vp]),
+        "ufsecp_gpu_ext_copy_to_device": (c_int, [vp, vp, vp, c_size_t]),
+        "ufsecp_gpu_ext_copy_to_host": (c_int, [vp, vp, vp, c_size_t]),
+        "ufsecp_gpu_ext_event_create": (c_int, [vp, POINTER(vp)]),
+        "ufsecp_gpu_ext_event_record": (c_int, [vp, vp, vp]),
+        "ufsecp_gpu_ext_event_elapsed": (c_int, [vp, vp, POINTER(ctypes.c_float)]),
+        "ufsecp_gpu_ext_event_destroy": (None, [vp]),
         "ufsecp_gpu_ext_stream": (vp, [vp]),
         "ufsecp_gpu_ext_synchronize": (c_int, [vp]),
         "ufsecp_gpu_ext_selftest": (c_int, [vp, c_int, vp, vp, vp, c_size_t]),
@@ -100,6 +124,12 @@ def _bind(lib: ctypes.CDLL) -> None:
         fn.restype = res
         fn.argtypes = args
     _ = u8p
+
+
+def _nitems(b, stride: int) -> int:
+    """Number of fixed-stride items in a buffer (byte size // stride)."""
+    nbytes = getattr(b, "nbytes", None)
+    return (nbytes if nbytes is not None else len(b)) // stride
 
 
 def _buf(b) -> c_void_p:
@@ -176,7 +206,7 @@ class UfsecpGpu:
     # -- batch verification ---------------------------------------------------
     def ecdsa_verify_batch(self, msgs, pubkeys33, sigs64, count: Optional[int] = None):
         import numpy as np
-        n = len(msgs) // 32 if count is None else count
+        n = _nitems(msgs, 32) if count is None else count
         out = np.zeros(n, dtype=np.uint8)
         rc = self._lib.ufsecp_gpu_ecdsa_verify_batch(self._ctx, _buf(msgs), _buf(pubkeys33), _buf(sigs64), n,
                                                      _buf(out))
@@ -185,7 +215,7 @@ class UfsecpGpu:
 
     def schnorr_verify_batch(self, msgs, pubkeys_x32, sigs64, count: Optional[int] = None):
         import numpy as np
-        n = len(msgs) // 32 if count is None else count
+        n = _nitems(msgs, 32) if count is None else count
         out = np.zeros(n, dtype=np.uint8)
         rc = self._lib.ufsecp_gpu_schnorr_verify_batch(self._ctx, _buf(msgs), _buf(pubkeys_x32), _buf(sigs64), n,
                                                        _buf(out))
@@ -193,18 +223,28 @@ class UfsecpGpu:
         return out
 
     def msm(self, scalars32, points33, n: Optional[int] = None) -> bytes:
-        n = len(scalars32) // 32 if n is None else n
+        n = _nitems(scalars32, 32) if n is None else n
         out = ctypes.create_string_buffer(33)
         rc = self._lib.ufsecp_gpu_msm(self._ctx, _buf(scalars32), _buf(points33), n, out)
         self._check("ufsecp_gpu_msm", rc)
         return out.raw
 
-    def msm_rc(self, scalars32, points33, n: Optional[int] = None):
-        """(return code, 33-byte output) without raising."""
-        n = len(scalars32) // 32 if n is None else n
+    def msm_rc(self, scalars32, points33, n: Optional[int] = None, window_bits: Optional[int] = None):
+        """(return code, 33-byte output) without raising; window_bits forces c."""
+        n = _nitems(scalars32, 32) if n is None else n
         out = ctypes.create_string_buffer(33)
-        rc = self._lib.ufsecp_gpu_msm(self._ctx, _buf(scalars32), _buf(points33), n, out)
+        if window_bits is None:
+            rc = self._lib.ufsecp_gpu_msm(self._ctx, _buf(scalars32), _buf(points33), n, out)
+        else:
+            rc = self._lib.ufsecp_gpu_ext_msm(self._ctx, _buf(scalars32), _buf(points33), n, out, window_bits)
         return rc, out.raw
+
+    def msm_device(self, d_scalars: int, d_points: int, n: int, d_out33: int, d_status: int, window_bits: int = 0,
+                   stream: Optional[int] = None) -> None:
+        rc = self._lib.ufsecp_gpu_ext_msm_device(self._ctx, c_void_p(d_scalars), c_void_p(d_points), n,
+                                                 c_void_p(d_out33), c_void_p(d_status), window_bits,
+                                                 c_void_p(stream))
+        self._check("ufsecp_gpu_ext_msm_device", rc)
 
     # -- device-resident variants (pointers are device addresses) --------------
     def schnorr_verify_device(self, d_msgs: int, d_pubx: int, d_sigs: int, count: int, d_out: int,
@@ -220,6 +260,35 @@ class UfsecpGpu:
                                                           c_void_p(d_sigs), count, c_void_p(d_out),
                                                           c_void_p(stream))
         self._check("ufsecp_gpu_ext_ecdsa_verify_device", rc)
+
+    @staticmethod
+    def batch_table(batches) -> ctypes.Array:
+        """ctypes array of VerifyBatchDesc from (kind, count, d_msgs, d_pubs, d_sigs, d_out) tuples."""
+        arr = (VerifyBatchDesc * len(batches))()
+        for i, (kind, count, m, p, s, o) in enumerate(batches):
+            arr[i] = VerifyBatchDesc(kind, count, m, p, s, o)
+        return arr
+
+    def verify_multi_device(self, table, nbatches: int, use_graph: bool = True,
+                            stream: Optional[int] = None) -> None:
+        rc = self._lib.ufsecp_gpu_ext_verify_multi_device(self._ctx, ctypes.cast(table, c_void_p), nbatches,
+                                                          1 if use_graph else 0, c_void_p(stream))
+        self._check("ufsecp_gpu_ext_verify_multi_device", rc)
+
+    def verify_each_device(self, table, nbatches: int, stream: Optional[int] = None) -> None:
+        rc = self._lib.ufsecp_gpu_ext_verify_each_device(self._ctx, ctypes.cast(table, c_void_p), nbatches,
+                                                         c_void_p(stream))
+        self._check("ufsecp_gpu_ext_verify_each_device", rc)
+
+    # -- device memory / events (library-owned HIP runtime) -------------------
+    def to_device(self, arr) -> "DeviceArray":
+        return DeviceArray.from_numpy(self, arr)
+
+    def empty(self, nbytes: int) -> "DeviceArray":
+        return DeviceArray(self, nbytes)
+
+    def event(self) -> "Event":
+        return Event(self)
 
     def synchronize(self) -> None:
         self._check("ufsecp_gpu_ext_synchronize", self._lib.ufsecp_gpu_ext_synchronize(self._ctx))
@@ -237,3 +306,90 @@ class UfsecpGpu:
         rc = self._lib.ufsecp_gpu_ext_selftest(self._ctx, op, _buf(a), _buf(b), _buf(out), n)
         self._check("ufsecp_gpu_ext_selftest", rc)
         return out
+
+
+class DeviceArray:
+    """Device buffer allocated through the library (ufsecp_gpu_ext_alloc)."""
+
+    def __init__(self, gpu: UfsecpGpu, nbytes: int):
+        self.gpu = gpu
+        self.nbytes = int(nbytes)
+        p = c_void_p()
+        rc = gpu.lib.ufsecp_gpu_ext_alloc(gpu.ctx, max(1, self.nbytes), byref(p))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_alloc", rc)
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, gpu: UfsecpGpu, arr) -> "DeviceArray":
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        d = cls(gpu, arr.nbytes)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr) -> None:
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if arr.nbytes:
+            rc = self.gpu.lib.ufsecp_gpu_ext_copy_to_device(self.gpu.ctx, c_void_p(self.ptr), _buf(arr), arr.nbytes)
+            if rc != UFSECP_OK:
+                raise UfsecpError("ufsecp_gpu_ext_copy_to_device", rc)
+
+    def download(self, dtype="uint8", count: Optional[int] = None):
+        import numpy as np
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dt)
+        if out.nbytes:
+            rc = self.gpu.lib.ufsecp_gpu_ext_copy_to_host(self.gpu.ctx, _buf(out), c_void_p(self.ptr), out.nbytes)
+            if rc != UFSECP_OK:
+                raise UfsecpError("ufsecp_gpu_ext_copy_to_host", rc)
+        return out
+
+    def at(self, offset: int) -> int:
+        """Device address `offset` bytes into the buffer."""
+        return self.ptr + offset
+
+    def free(self) -> None:
+        if self.ptr:
+            self.gpu.lib.ufsecp_gpu_ext_free(self.gpu.ctx, c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if self.ptr and self.gpu.ctx:
+                self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """hipEvent recorded on a stream (default: the context stream)."""
+
+    def __init__(self, gpu: UfsecpGpu):
+        self.gpu = gpu
+        p = c_void_p()
+        rc = gpu.lib.ufsecp_gpu_ext_event_create(gpu.ctx, byref(p))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_event_create", rc)
+        self.ptr = p.value
+
+    def record(self, stream: Optional[int] = None) -> None:
+        rc = self.gpu.lib.ufsecp_gpu_ext_event_record(self.gpu.ctx, c_void_p(self.ptr), c_void_p(stream))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_event_record", rc)
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = ctypes.c_float()
+        rc = self.gpu.lib.ufsecp_gpu_ext_event_elapsed(c_void_p(self.ptr), c_void_p(stop.ptr), byref(ms))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_event_elapsed", rc)
+        return ms.value
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.gpu.lib.ufsecp_gpu_ext_event_destroy(c_void_p(self.ptr))
+        except Exception:
+            pass

@@ -83,6 +83,13 @@ GpuError RocmBackend::fail(GpuError e, const char* what, hipError_t he) {
 RocmBackend::~RocmBackend() {
     if (device_ >= 0) (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (graph_stream_) (void)hipStreamDestroy(graph_stream_);
+    d_descs_.release();
+    d_tiles_.release();
+    d_words_.release();
+    h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
     h_stage_.release();
@@ -194,6 +201,98 @@ GpuError RocmBackend::schnorr_verify_device(const uint8_t* d_msgs, const uint8_t
     hipError_t e = launch_schnorr_verify(d_msgs, d_pubx, d_sigs, (uint32_t)count, gdesc_, d_mid_, d_out, false,
                                          st ? st : stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "Schnorr verify launch", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::verify_each_device(const BatchDesc* batches, size_t nbatches, hipStream_t st) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    st = st ? st : stream_;
+    for (size_t b = 0; b < nbatches; ++b) {
+        const BatchDesc& d = batches[b];
+        if (d.count == 0) continue;
+        hipError_t e = d.kind == 0
+                           ? launch_ecdsa_verify(d.msgs, d.pubs, d.sigs, d.count, gdesc_, d.out, false, st)
+                           : launch_schnorr_verify(d.msgs, d.pubs, d.sigs, d.count, gdesc_, d_mid_, d.out, false, st);
+        if (e != hipSuccess) return fail(GpuError::Launch, "per-batch verify launch", e);
+    }
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatches, bool use_graph,
+                                          hipStream_t st) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    st = st ? st : stream_;
+    // tile list: 64-item tiles per batch, in batch order
+    size_t ntiles = 0;
+    for (size_t b = 0; b < nbatches; ++b) {
+        if (batches[b].kind > 1) return fail(GpuError::BadInput, "batch kind must be 0 (ECDSA) or 1 (BIP-340)");
+        ntiles += (batches[b].count + kTileItems - 1) / kTileItems;
+    }
+    if (ntiles == 0) {
+        clear_error();
+        return GpuError::Ok;
+    }
+    if (ntiles > 0xFFFFFFFFull) return fail(GpuError::BadInput, "too many tiles");
+    const size_t desc_bytes = nbatches * sizeof(BatchDesc), tile_bytes = ntiles * sizeof(TileDesc);
+    hipError_t e;
+    if ((e = h_tables_.ensure(desc_bytes + tile_bytes + 64)) != hipSuccess ||
+        (e = d_descs_.ensure(desc_bytes)) != hipSuccess || (e = d_tiles_.ensure(tile_bytes)) != hipSuccess ||
+        (e = d_words_.ensure(64)) != hipSuccess)
+        return fail(GpuError::Memory, "multi-batch tables", e);
+    // The pinned staging area may still feed an earlier copy on this stream.
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(GpuError::Backend, "staging wait", e);
+    uint8_t* h = (uint8_t*)h_tables_.ptr;
+    std::memcpy(h, batches, desc_bytes);
+    TileDesc* tl = (TileDesc*)(h + desc_bytes);
+    size_t t = 0;
+    for (size_t b = 0; b < nbatches; ++b)
+        for (uint32_t f = 0; f < batches[b].count; f += kTileItems) tl[t++] = TileDesc{(uint32_t)b, f};
+    uint32_t* words = (uint32_t*)(h + desc_bytes + tile_bytes);
+    words[0] = (uint32_t)ntiles;
+    uint32_t* d_words = (uint32_t*)d_words_.ptr;   // [0] = ntiles, [16] = work counter
+    if ((e = hipMemcpyAsync(d_descs_.ptr, h, desc_bytes, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_tiles_.ptr, tl, tile_bytes, hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_words, words, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return fail(GpuError::Memory, "copy multi-batch tables", e);
+    if (multi_grid_ == 0) {
+        hipDeviceProp_t p;
+        if ((e = hipGetDeviceProperties(&p, device_)) != hipSuccess) return fail(GpuError::Device, "properties", e);
+        cus_ = p.multiProcessorCount;
+        multi_grid_ = (unsigned)(cus_ * verify_multi_blocks_per_cu());
+    }
+    const BatchDesc* dd = (const BatchDesc*)d_descs_.ptr;
+    const TileDesc* dt = (const TileDesc*)d_tiles_.ptr;
+    if (!use_graph) {
+        if ((e = launch_u32_fill(d_words + 16, 0u, 1, st)) != hipSuccess ||
+            (e = launch_verify_multi(dd, dt, d_words, d_words + 16, gdesc_, d_mid_, multi_grid_, st)) != hipSuccess)
+            return fail(GpuError::Launch, "multi-batch verify launch", e);
+        clear_error();
+        return GpuError::Ok;
+    }
+    // Graph: [counter reset (fill kernel), persistent verify].  Captured once and
+    // re-captured only when the table buffers move.
+    if (!graph_exec_ || graph_descs_ != dd || graph_tiles_ != dt) {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+        if (!graph_stream_ && (e = hipStreamCreateWithFlags(&graph_stream_, hipStreamNonBlocking)) != hipSuccess)
+            return fail(GpuError::Queue, "capture stream", e);
+        if ((e = hipStreamBeginCapture(graph_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return fail(GpuError::Backend, "hipStreamBeginCapture", e);
+        hipError_t e1 = launch_u32_fill(d_words + 16, 0u, 1, graph_stream_);
+        hipError_t e2 = launch_verify_multi(dd, dt, d_words, d_words + 16, gdesc_, d_mid_, multi_grid_, graph_stream_);
+        e = hipStreamEndCapture(graph_stream_, &graph_);
+        if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess)
+            return fail(GpuError::Backend, "graph capture", e != hipSuccess ? e : (e1 != hipSuccess ? e1 : e2));
+        if ((e = hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0)) != hipSuccess)
+            return fail(GpuError::Backend, "hipGraphInstantiate", e);
+        graph_descs_ = dd;
+        graph_tiles_ = dt;
+    }
+    if ((e = hipGraphLaunch(graph_exec_, st)) != hipSuccess) return fail(GpuError::Launch, "hipGraphLaunch", e);
     clear_error();
     return GpuError::Ok;
 }

@@ -86,6 +86,11 @@ public:
     GpuError schnorr_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubx, const uint8_t* d_sigs,
                                    size_t count, uint8_t* d_out, hipStream_t st);
 
+    // Many uneven batches in one persistent launch (optionally replayed from a
+    // captured hipGraph); `each` issues one plain launch per batch instead.
+    GpuError verify_multi_device(const BatchDesc* batches, size_t nbatches, bool use_graph, hipStream_t st);
+    GpuError verify_each_device(const BatchDesc* batches, size_t nbatches, hipStream_t st);
+
     GpuError selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t count);
     GpuError synchronize();
 
@@ -111,6 +116,16 @@ private:
     DeviceBuffer d_in_[3];
     DeviceBuffer d_out_;
     PinnedBuffer h_stage_;
+    // multi-batch state: descriptor/tile tables, work counter, captured graph
+    DeviceBuffer d_descs_, d_tiles_, d_words_;
+    PinnedBuffer h_tables_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipStream_t graph_stream_ = nullptr;
+    const void* graph_descs_ = nullptr;
+    const void* graph_tiles_ = nullptr;
+    unsigned multi_grid_ = 0;
+    int cus_ = 0;
     GpuError last_err_ = GpuError::Ok;
     std::string last_msg_ = "OK";
 };

@@ -40,6 +40,32 @@ hipError_t launch_ecdsa_verify(const uint8_t* msgs, const uint8_t* pubs33, const
 hipError_t launch_schnorr_verify(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, uint32_t count,
                                  const GcombDesc& g, const uint32_t* mid, uint8_t* out, bool collect,
                                  hipStream_t st);
+// One verify batch of a multi-batch launch (device pointers; strides as in
+// the single-batch ABI: msgs 32, pubkeys 33 (ECDSA) or 32 (BIP-340), sigs 64).
+struct BatchDesc {
+    uint32_t kind;    // 0 = ECDSA, 1 = BIP-340
+    uint32_t count;
+    const uint8_t* msgs;
+    const uint8_t* pubs;
+    const uint8_t* sigs;
+    uint8_t* out;
+};
+
+// 64-item tile of one batch.
+struct TileDesc {
+    uint32_t batch;
+    uint32_t first;
+};
+
+constexpr uint32_t kTileItems = 64;
+
+// ntiles and counter are device words so a captured graph stays valid when
+// the tile list changes between replays.
+hipError_t launch_verify_multi(const BatchDesc* descs, const TileDesc* tiles, const uint32_t* ntiles,
+                               uint32_t* counter, const GcombDesc& g, const uint32_t* mid, unsigned grid_blocks,
+                               hipStream_t st);
+int verify_multi_blocks_per_cu();
+hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,

@@ -189,6 +189,49 @@ k_schnorr_verify(const u8* __restrict__ msgs, const u8* __restrict__ pubx,
     }
 }
 
+// Persistent multi-batch verification: each wave pulls 64-item tiles of
+// uneven batches (ECDSA or BIP-340) from a device work counter until the tile
+// list is exhausted.  A tile never straddles batches, so every wave runs one
+// signature kind.  The loop ends for every wave once the counter passes
+// ntiles, so the grid drains before the kernel returns.
+__global__ void __launch_bounds__(kVerifyBlock)
+k_verify_multi(const BatchDesc* __restrict__ descs, const TileDesc* __restrict__ tiles,
+               const u32* __restrict__ ntiles_p, u32* __restrict__ counter, gcomb G, const u32* __restrict__ mid_g) {
+    const u32 lane = threadIdx.x & 63u;
+    const u32 ntiles = *ntiles_p;
+    u32 mid[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mid[j] = mid_g[j];
+    for (;;) {
+        u32 t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1u);
+        t = __shfl(t, 0, 64);
+        if (t >= ntiles) break;
+        const TileDesc tl = tiles[t];
+        const BatchDesc d = descs[tl.batch];
+        const u32 i = tl.first + lane;
+        if (i >= d.count) continue;
+        bool v;
+        if (d.kind == 0) {
+            u32 z[8], r[8], s[8], x[8];
+            load_be32_aligned(z, d.msgs + 32ull * i);
+            load_be32_aligned(r, d.sigs + 64ull * i);
+            load_be32_aligned(s, d.sigs + 64ull * i + 32);
+            const u8* pk = d.pubs + 33ull * i;
+            load_be32_bytes(x, pk + 1);
+            v = ecdsa_verify_one(z, r, s, pk[0], x, G);
+        } else {
+            u32 m[8], px[8], r[8], s[8];
+            load_be_words(m, d.msgs + 32ull * i);
+            load_be_words(px, d.pubs + 32ull * i);
+            load_be_words(r, d.sigs + 64ull * i);
+            load_be_words(s, d.sigs + 64ull * i + 32);
+            v = schnorr_verify_one(m, px, r, s, mid, G);
+        }
+        d.out[i] = v ? 1 : 0;
+    }
+}
+
 // entry(i, j) = (2j+1) * 2^(w*i) * G for i < nwin; entry(nwin, 0) = 2^(w*nwin) G.
 __global__ void __launch_bounds__(64) k_build_gcomb(aff* tab, int w, int nwin) {
     const int half = 1 << (w - 1);
@@ -327,6 +370,22 @@ hipError_t launch_schnorr_verify(const u8* msgs, const u8* pubx, const u8* sigs,
         hipLaunchKernelGGL(k_schnorr_verify<0>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
                            msgs, pubx, sigs, count, G, mid, out);
     return hipGetLastError();
+}
+
+hipError_t launch_verify_multi(const BatchDesc* descs, const TileDesc* tiles, const uint32_t* ntiles,
+                               uint32_t* counter, const GcombDesc& g, const uint32_t* mid, unsigned grid_blocks,
+                               hipStream_t st) {
+    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
+    hipLaunchKernelGGL(k_verify_multi, dim3(grid_blocks), dim3(kVerifyBlock), 0, st, descs, tiles, ntiles, counter,
+                       G, mid);
+    return hipGetLastError();
+}
+
+int verify_multi_blocks_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify_multi, kVerifyBlock, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
 }
 
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st) {

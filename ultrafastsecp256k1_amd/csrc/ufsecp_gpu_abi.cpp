@@ -4,6 +4,7 @@
 // reference C ABI shim src/cpu/src/ufsecp_gpu_impl.cpp:76-140 (helpers),
 // 182-338 (lifecycle + ECDSA), 378-399 (Schnorr), 453-493 (collect),
 // 541-558 (MSM), 1136-1158 (error strings).
+#include <cstddef>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -263,6 +264,136 @@ ufsecp_error_t ufsecp_gpu_ext_schnorr_verify_device(ufsecp_gpu_ctx* ctx, const u
                                                           d_out_results, (hipStream_t)stream));
     }
     UFSECP_GPU_CATCH
+}
+
+static_assert(sizeof(ufsecp_gpu_verify_batch_t) == sizeof(ufs::BatchDesc), "batch descriptor layout");
+static_assert(offsetof(ufsecp_gpu_verify_batch_t, out_results) == offsetof(ufs::BatchDesc, out),
+              "batch descriptor layout");
+
+static ufsecp_error_t check_batches(const ufsecp_gpu_verify_batch_t* b, size_t nb) {
+    if (nb > 0 && !b) return UFSECP_ERR_NULL_ARG;
+    size_t total = 0;
+    for (size_t i = 0; i < nb; ++i) {
+        if (b[i].kind > 1) return UFSECP_ERR_BAD_INPUT;
+        if (b[i].count == 0) continue;
+        if (!b[i].msg_hashes32 || !b[i].pubkeys || !b[i].sigs64 || !b[i].out_results) return UFSECP_ERR_NULL_ARG;
+        total += b[i].count;
+        if (total > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    }
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_verify_multi_device(ufsecp_gpu_ctx* ctx, const ufsecp_gpu_verify_batch_t* batches,
+                                                  size_t nbatches, int use_graph, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    const ufsecp_error_t c = check_batches(batches, nbatches);
+    if (c != UFSECP_OK) return c;
+    try {
+        return to_abi(ctx->backend->verify_multi_device(reinterpret_cast<const ufs::BatchDesc*>(batches), nbatches,
+                                                        use_graph != 0, (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_verify_each_device(ufsecp_gpu_ctx* ctx, const ufsecp_gpu_verify_batch_t* batches,
+                                                 size_t nbatches, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    const ufsecp_error_t c = check_batches(batches, nbatches);
+    if (c != UFSECP_OK) return c;
+    try {
+        return to_abi(ctx->backend->verify_each_device(reinterpret_cast<const ufs::BatchDesc*>(batches), nbatches,
+                                                       (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_msm(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32, const uint8_t* points33, size_t n,
+                                  uint8_t* out_result33, int window_bits) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN || window_bits < 0 || window_bits == 1 || window_bits > 20) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_result33, 1, 33)) return UFSECP_ERR_BAD_INPUT;
+    if (!scalars32 || !points33 || !out_result33) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->msm->msm_host(scalars32, points33, n, out_result33, window_bits), out_result33, 1,
+                            33);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32, const uint8_t* d_points33,
+                                         size_t n, uint8_t* d_out33, int32_t* d_status, int window_bits,
+                                         void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN || window_bits < 0 || window_bits == 1 || window_bits > 20) return UFSECP_ERR_BAD_INPUT;
+    if (!d_scalars32 || !d_points33 || !d_out33 || !d_status) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->msm->msm_device(d_scalars32, d_points33, n, d_out33, d_status, window_bits,
+                                           (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** d_ptr_out) {
+    if (!ctx || !d_ptr_out) return UFSECP_ERR_NULL_ARG;
+    *d_ptr_out = nullptr;
+    if (bytes == 0) return UFSECP_ERR_BAD_INPUT;
+    if (hipMalloc(d_ptr_out, bytes) != hipSuccess) {
+        *d_ptr_out = nullptr;
+        return UFSECP_ERR_GPU_MEMORY;
+    }
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_free(ufsecp_gpu_ctx* ctx, void* d_ptr) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (!d_ptr) return UFSECP_OK;
+    (void)hipStreamSynchronize(ctx->backend->stream());
+    return hipFree(d_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_copy_to_device(ufsecp_gpu_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+    if (!ctx || (bytes && (!d_dst || !h_src))) return UFSECP_ERR_NULL_ARG;
+    hipStream_t st = ctx->backend->stream();
+    if (hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return UFSECP_ERR_GPU_MEMORY;
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_copy_to_host(ufsecp_gpu_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+    if (!ctx || (bytes && (!d_src || !h_dst))) return UFSECP_ERR_NULL_ARG;
+    hipStream_t st = ctx->backend->stream();
+    if (hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return UFSECP_ERR_GPU_MEMORY;
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_event_create(ufsecp_gpu_ctx* ctx, void** event_out) {
+    if (!ctx || !event_out) return UFSECP_ERR_NULL_ARG;
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return UFSECP_ERR_GPU_BACKEND;
+    *event_out = (void*)ev;
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_event_record(ufsecp_gpu_ctx* ctx, void* event, void* stream) {
+    if (!ctx || !event) return UFSECP_ERR_NULL_ARG;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->backend->stream();
+    return hipEventRecord((hipEvent_t)event, st) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_BACKEND;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_event_elapsed(void* start, void* stop, float* ms_out) {
+    if (!start || !stop || !ms_out) return UFSECP_ERR_NULL_ARG;
+    if (hipEventSynchronize((hipEvent_t)stop) != hipSuccess) return UFSECP_ERR_GPU_BACKEND;
+    return hipEventElapsedTime(ms_out, (hipEvent_t)start, (hipEvent_t)stop) == hipSuccess ? UFSECP_OK
+                                                                                          : UFSECP_ERR_GPU_BACKEND;
+}
+
+void ufsecp_gpu_ext_event_destroy(void* event) {
+    if (event) (void)hipEventDestroy((hipEvent_t)event);
 }
 
 void* ufsecp_gpu_ext_stream(ufsecp_gpu_ctx* ctx) {

@@ -176,6 +176,37 @@ def generate_msm(n: int, seed: int = SEED_MSM, first: int = 0, threads: int = 0)
     return sc, pts, ks
 
 
+def mix_plan(nbatches: int, max_size: int, seed: int = SEED_MIX, ecdsa_share: float = 0.7):
+    """Uneven batch mix: sizes log-uniform in [1, max_size], kind ECDSA with
+    probability ecdsa_share (else BIP-340).  Returns [(kind, size)] with kind
+    0 = ECDSA, 1 = BIP-340."""
+    import math
+    r = SplitMix64(seed, 0xFFFFFFFF)
+    plan = []
+    for _ in range(nbatches):
+        u = r.next() / 2.0 ** 64
+        size = int(math.exp(u * math.log(max_size + 1)))
+        size = min(max(size, 1), max_size)
+        kind = 0 if (r.next() % 1000) < int(ecdsa_share * 1000) else 1
+        plan.append((kind, size))
+    return plan
+
+
+def generate_mix(plan, seed: int = SEED_MIX, threads: int = 0):
+    """Inputs for a mix plan: one ECDSA and one BIP-340 pool (items of the
+    ECDSA / BIP-340 workloads with seeds seed+1 / seed+2) and, per batch, its
+    (kind, first item in the pool, size)."""
+    ne = sum(s for k, s in plan if k == 0)
+    ns = sum(s for k, s in plan if k == 1)
+    pools = (generate("ecdsa", ne, seed + 1, threads=threads), generate("schnorr", ns, seed + 2, threads=threads))
+    offs = [0, 0]
+    layout = []
+    for k, s in plan:
+        layout.append((k, offs[k], s))
+        offs[k] += s
+    return pools, layout
+
+
 def msm_closed_form_scalar(scalars: np.ndarray, ks: np.ndarray) -> int:
     """sum s_i * k_i mod n (the MSM result is this scalar times G)."""
     acc = 0

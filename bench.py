@@ -76,42 +76,112 @@ def cpu_baseline(kind: str, batch, cores: int, budget_items: int, gpu_verdicts=N
     return out
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="schnorr", choices=["schnorr", "ecdsa"])
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--cpu-items", type=int, default=1 << 18)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the verify path shards by item range with no communication, see DESIGN.md", file=sys.stderr)
-        return 2
-
+def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) -> dict:
+    """Uneven batching: persistent multi-batch launch (direct and graph replay)
+    vs one plain launch per batch, same inputs, verdicts compared."""
     import numpy as np
-    from ultrafastsecp256k1_amd import UfsecpGpu
     from ultrafastsecp256k1_amd import workload as wl
+    t0 = time.perf_counter()
+    plan = wl.mix_plan(nbatches, max_size)
+    (pe, ps), layout = wl.generate_mix(plan, threads=16)
+    log(f"[bench] mix: {nbatches} batches, {pe.n} ECDSA + {ps.n} BIP-340 items generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    pools = [tuple(gpu.to_device(a) for a in (p.msgs, p.pubkeys, p.sigs)) for p in (pe, ps)]
+    outs = [gpu.empty(max(1, p.n)) for p in (pe, ps)]
+    pstride = (33, 32)
+    batches = [(k, s, pools[k][0].at(32 * f), pools[k][1].at(pstride[k] * f), pools[k][2].at(64 * f),
+                outs[k].at(f)) for k, f, s in layout]
+    table = gpu.batch_table(batches)
+    sub = gpu.batch_table(batches[:plain_batches])
+    sub_items = sum(s for _, _, s in layout[:plain_batches])
+    total = pe.n + ps.n
 
-    kind, n = args.workload, args.n
+    def timed(fn, r):
+        fn()
+        gpu.synchronize()
+        a, b = gpu.event(), gpu.event()
+        a.record()
+        for _ in range(r):
+            fn()
+        b.record()
+        gpu.synchronize()
+        return a.elapsed_ms(b) / r
+
+    ms_graph = timed(lambda: gpu.verify_multi_device(table, len(batches), use_graph=True), reps)
+    got = [o.download() for o in outs]
+    corr = [pe.corrupted.astype(bool), ps.corrupted.astype(bool)]
+    parity = all(bool(np.all(got[k][:len(corr[k])][~corr[k]] == 1) and np.all(got[k][:len(corr[k])][corr[k]] == 0))
+                 for k in (0, 1))
+    ms_direct = timed(lambda: gpu.verify_multi_device(table, len(batches), use_graph=False), reps)
+    ms_sub_persist = timed(lambda: gpu.verify_multi_device(sub, plain_batches, use_graph=True), reps)
+    for o in outs:
+        o.upload(np.zeros(o.nbytes, np.uint8))
+    ms_sub_plain = timed(lambda: gpu.verify_each_device(sub, plain_batches), 1)
+    got2 = [o.download() for o in outs]
+    same = True
+    # plain launches covered the first `plain_batches` batches: compare those verdicts
+    for k, f, s in layout[:plain_batches]:
+        same &= bool(np.array_equal(got2[k][f:f + s], got[k][f:f + s]))
+    return {"batches": nbatches, "items": total, "ecdsa_items": pe.n, "schnorr_items": ps.n,
+            "persistent_graph_ms": round(ms_graph, 3), "persistent_direct_ms": round(ms_direct, 3),
+            "persistent_sigs_per_s": total / (ms_graph * 1e-3),
+            "subset_batches": plain_batches, "subset_items": sub_items,
+            "subset_persistent_ms": round(ms_sub_persist, 3), "subset_plain_per_batch_ms": round(ms_sub_plain, 3),
+            "speedup_persistent_vs_plain": ms_sub_plain / ms_sub_persist,
+            "parity": parity, "plain_matches_persistent": same}
+
+
+def bench_msm(gpu, n: int, windows, reps: int) -> dict:
+    """Pippenger MSM over n terms with bases k_i*G: c sweep, device-resident
+    inputs, result checked against the closed form (sum s_i k_i) G."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    from ec_math import compress, mul
+    from ultrafastsecp256k1_amd import workload as wl
+    t0 = time.perf_counter()
+    sc, pts, ks = wl.generate_msm(n, threads=16)
+    expect = compress(mul(wl.msm_closed_form_scalar(sc, ks)))
+    log(f"[bench] msm: {n} terms generated in {time.perf_counter() - t0:.1f}s")
+    d_s, d_p = gpu.to_device(sc), gpu.to_device(pts)
+    d_o = gpu.empty(64)
+    res = {}
+    for c in windows:
+        gpu.msm_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c)
+        gpu.synchronize()
+        a, b = gpu.event(), gpu.event()
+        a.record()
+        for _ in range(reps):
+            gpu.msm_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c)
+        b.record()
+        ms = a.elapsed_ms(b) / reps
+        out = d_o.download()
+        status = int(np.frombuffer(out[48:52].tobytes(), np.int32)[0])
+        res[str(c)] = {"ms": round(ms, 3), "ok": status == 0 and out[:33].tobytes() == expect}
+        log(f"[bench] msm n={n} c={c}: {ms:.2f} ms ok={res[str(c)]['ok']}")
+    best = min(res, key=lambda c: res[c]["ms"])
+    return {"n": n, "sweep": res, "best_c": int(best), "best_ms": res[best]["ms"],
+            "terms_per_s": n / (res[best]["ms"] * 1e-3), "parity": all(v["ok"] for v in res.values())}
+
+
+def run_verify(gpu, kind: str, n: int, steps: int, warmup: int) -> dict:
+    """One batch of n signatures resident in HBM, timed per launch with HIP
+    events on the context stream (the stream the kernel runs on)."""
+    import numpy as np
+    from ultrafastsecp256k1_amd import workload as wl
     t0 = time.perf_counter()
     batch = wl.generate(kind, n, threads=16)
     log(f"[bench] generated {n} {kind} items in {time.perf_counter() - t0:.1f}s")
-    gpu = UfsecpGpu()
     d_m, d_p, d_s = (gpu.to_device(a) for a in (batch.msgs, batch.pubkeys, batch.sigs))
     d_o = gpu.empty(n)
     launch = gpu.schnorr_verify_device if kind == "schnorr" else gpu.ecdsa_verify_device
 
-    # All launches and events go to the context's HIP stream.
     def step():
         launch(d_m.ptr, d_p.ptr, d_s.ptr, n, d_o.ptr, None)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     gpu.synchronize()
-    evs = [(gpu.event(), gpu.event()) for _ in range(args.steps)]
+    evs = [(gpu.event(), gpu.event()) for _ in range(steps)]
     t0 = time.perf_counter()
     for a, b in evs:
         a.record()
@@ -121,42 +191,87 @@ def main() -> int:
     wall = time.perf_counter() - t0
     kern_ms = [a.elapsed_ms(b) for a, b in evs]
     verdicts = d_o.download()
-    # parity: every untouched signature verifies, every corrupted one fails
     corr = batch.corrupted.astype(bool)
-    parity_ok = bool(np.all(verdicts[~corr] == 1) and np.all(verdicts[corr] == 0))
-    ms_step = wall * 1e3 / args.steps
-    value = n / (wall / args.steps)
-    kern_avg = sum(kern_ms) / len(kern_ms)
-    log(f"[bench] {kind} n={n}: {ms_step:.2f} ms/step, kernel {kern_avg:.2f} ms, {value / 1e6:.2f} M sigs/s, "
-        f"parity {parity_ok}")
+    parity = bool(np.all(verdicts[~corr] == 1) and np.all(verdicts[corr] == 0))
+    kern = sum(kern_ms) / len(kern_ms)
+    res = {"kind": kind, "n": n, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern,
+           "kernel_ms_min": min(kern_ms), "sigs_per_s": n / (wall / steps),
+           "kernel_sigs_per_s": n / (kern * 1e-3), "parity": parity}
+    log(f"[bench] {kind} n={n}: {res['ms_per_step']:.2f} ms/step, kernel {kern:.2f} ms, "
+        f"{res['sigs_per_s'] / 1e6:.2f} M sigs/s, parity {parity}")
+    return res, batch, verdicts
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="schnorr", choices=["schnorr", "ecdsa"])
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-items", type=int, default=1 << 18)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="ecdsa,mix,msm",
+                    help="extra measurements after the headline: ecdsa, mix (uneven batches), msm")
+    ap.add_argument("--mix-batches", type=int, default=10000)
+    ap.add_argument("--mix-plain-batches", type=int, default=500)
+    ap.add_argument("--msm-n", type=int, default=1 << 20)
+    ap.add_argument("--msm-windows", default="10,11,12,13,14,15,16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the verify path shards by item range with no communication, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    from ultrafastsecp256k1_amd import UfsecpGpu
+
+    kind, n = args.workload, args.n
+    t0 = time.perf_counter()
+    gpu = UfsecpGpu()
+    ctx_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] context created in {ctx_ms:.0f} ms")
+    head, batch, verdicts = run_verify(gpu, kind, n, args.steps, args.warmup)
 
     peak = measure_valu_peak()
     mad = peak["v_mad_u64_u32"]["ops_per_s"]
     w = W_SCHNORR if kind == "schnorr" else W_ECDSA
-    achieved = (n / (kern_avg * 1e-3)) * w
+    achieved = head["kernel_sigs_per_s"] * w
     roofline = {"bound": "valu", "unit": "TMAC32/s", "achieved": achieved / 1e12, "peak": mad / 1e12,
                 "frac": achieved / mad, "traffic": None,
-                "kernel": f"k_{kind}_verify<0>", "kernel_ms": round(kern_avg, 4),
+                "kernel": f"k_{kind}_verify<0>", "kernel_ms": round(head["kernel_ms"], 4),
                 "work_per_item_mac32": w,
-                "hbm_gbps": n * (BYTES_SCHNORR if kind == "schnorr" else BYTES_ECDSA) / (kern_avg * 1e-3) / 1e9,
+                "hbm_gbps": n * (BYTES_SCHNORR if kind == "schnorr" else BYTES_ECDSA) / (head["kernel_ms"] * 1e-3) / 1e9,
                 "peak_source": "valu_peak probe (v_mad_u64_u32, measured on this GPU)"}
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(kind, batch, cores, args.cpu_items, verdicts)
         log(f"[bench] cpu baseline: {cpu}")
+    del batch, verdicts
+    legs = {}
+    want = [x for x in args.legs.split(",") if x]
+    if "ecdsa" in want and kind != "ecdsa":
+        r, _, _ = run_verify(gpu, "ecdsa", n, max(3, args.steps // 2), 1)
+        r["valu_frac"] = r["kernel_sigs_per_s"] * W_ECDSA / mad
+        legs["ecdsa"] = r
+    if "mix" in want:
+        legs["mix"] = bench_mix(gpu, args.mix_batches, 4096, args.mix_plain_batches, 3)
+        log(f"[bench] mix: {legs['mix']}")
+    if "msm" in want:
+        legs["msm"] = bench_msm(gpu, args.msm_n, [int(c) for c in args.msm_windows.split(",")], 3)
+    parity = head["parity"] and all(v.get("parity", True) for v in legs.values())
     out = {
-        "metric": METRIC, "value": value, "unit": "sigs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": head["sigs_per_s"], "unit": "sigs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"{kind} batch verify, {n} signatures in one launch, 1% corrupted",
                    "batch": n, "kind": kind},
-        "parity": parity_ok, "roofline": roofline, "cpu_baseline": cpu,
+        "parity": parity, "roofline": roofline, "cpu_baseline": cpu, "legs": legs, "ctx_create_ms": round(ctx_ms, 1),
         "valu_peak": {k: v for k, v in peak.items() if k != "device"},
         "device": peak["device"],
     }
     print(json.dumps(out), flush=True)
-    return 0 if parity_ok else 1
+    return 0 if parity else 1
 
 
 if __name__ == "__main__":

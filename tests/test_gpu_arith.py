@@ -120,3 +120,17 @@ def test_variable_mul_glv(gpu):
     # lambda * P == (beta * x, y)
     assert mul(LAMBDA, pts[3]) == (BETA * pts[3][0] % P, pts[3][1])
     _ = (G, add)
+
+
+def test_field_mul_sqr_chains(gpu):
+    """4096 dependent multiplications / squarings per lane (the hot carry chains
+    back to back) agree with x*y^4096 and x^(2^4096)."""
+    xs = _vals(60, 9)
+    ys = list(reversed(_vals(60, 10)))
+    a, b = _pack(xs), _pack(ys)
+    out = gpu.selftest(6, a, b)
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        assert from_limbs(out[i, :8]) == x * pow(y, 4096, P) % P
+    out = gpu.selftest(7, a, b)
+    for i, x in enumerate(xs):
+        assert from_limbs(out[i, :8]) == pow(x, 1 << 4096, P)

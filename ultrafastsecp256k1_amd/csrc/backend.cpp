@@ -116,8 +116,8 @@ GpuError RocmBackend::init(uint32_t device_index) {
     e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(GpuError::Queue, "hipStreamCreate", e);
 
-    // Fixed-base comb table for G: window width from UFSECP_GCOMB_W (4..12).
-    const int w = env_int("UFSECP_GCOMB_W", 8, 4, 12);
+    // Fixed-base comb table for G: window width from UFSECP_GCOMB_W (4..16).
+    const int w = env_int("UFSECP_GCOMB_W", 16, 4, 16);
     const size_t entries = gcomb_entries(w);
     e = hipMalloc(&d_gcomb_, entries * 64);
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);

@@ -28,6 +28,8 @@ enum SelftestOp : int {
     kOpFeSub = 3,
     kOpFeInv = 4,
     kOpFeSqrt = 5,
+    kOpFeMulChain = 6,
+    kOpFeSqrChain = 7,
     kOpScMul = 10,
     kOpScInv = 11,
     kOpGlv = 12,

@@ -305,6 +305,18 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
     case kOpFeSub: fe_sub(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
     case kOpFeInv: fe_inv(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
     case kOpFeSqrt: { bool ok = fe_sqrt(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; o[8] = ok; } break;
+    case kOpFeMulChain:   // throughput probe: 4096 dependent multiplications
+        r = x;
+        for (int k = 0; k < 4096; ++k) fe_mul(r, r, y);
+        fe_normalize(r);
+        for (int l = 0; l < 8; ++l) o[l] = r.v[l];
+        break;
+    case kOpFeSqrChain:   // throughput probe: 4096 dependent squarings
+        r = x;
+        for (int k = 0; k < 4096; ++k) fe_sqr(r, r);
+        fe_normalize(r);
+        for (int l = 0; l < 8; ++l) o[l] = r.v[l];
+        break;
     case kOpScMul: sc_mul(sr, sx, sy); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
     case kOpScInv: sc_inv(sr, sx); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
     case kOpGlv: {

@@ -10,6 +10,7 @@ namespace ufs {
 
 constexpr int kMsmBlock = 256;
 constexpr int kReduceBlock = 256;
+constexpr uint32_t kSegChunk = 32;   // entries per lane in the bucket reduction
 
 UFS_DEV void msm_load_scalar(u32 k[8], const u8* p) {
     const uint4* q = reinterpret_cast<const uint4*>(p);
@@ -81,8 +82,8 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
 }
 
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* __restrict__ entries, int c,
-              int W) {
+k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* __restrict__ entries,
+              u32* __restrict__ keys, int c, int W) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const sc s = scl[i];
@@ -94,35 +95,124 @@ k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* 
             const u32 b = w * B + (u32)(d < 0 ? -d : d) - 1u;
             const u32 pos = atomicAdd(&cursor[b], 1u);
             entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
+            keys[pos] = b;
         }
     }
 }
 
+constexpr u32 kEmpty = 0xFFFFFFFFu;
+
+__global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p, u32 n) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    jac v;
+    jac_set_inf(v);
+    p[i] = v;
+}
+
+// Bucket accumulation, level 0: the bucket-sorted entries are cut into chunks
+// of K; lane j sums chunk j run by run.  A bucket that starts and ends inside
+// the chunk is written to bsum directly; the chunk's first bucket, if it began
+// in an earlier chunk, becomes the chunk's head partial, and its last bucket,
+// if it continues into the next chunk, its tail partial.  Partials are merged
+// by the higher levels, so every lane does at most K additions whatever the
+// bucket sizes (the top window's buckets can hold half of all terms).
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_buckets(const aff* __restrict__ pts, const u32* __restrict__ offsets, const u32* __restrict__ entries,
-              jac* __restrict__ bsum, u32 nb) {
-    const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const u32 beg = offsets[b], end = offsets[b + 1];
-    jac acc;
-    jac_set_inf(acc);
-    aff nxt;
-    u32 ent = 0;
-    if (beg < end) {
-        ent = entries[beg];
-        nxt = pts[ent & 0x7fffffffu];
-    }
-    for (u32 e = beg; e < end; ++e) {
-        aff P = nxt;
-        const bool neg = (ent >> 31) != 0;
-        if (e + 1 < end) {
-            ent = entries[e + 1];
-            nxt = pts[ent & 0x7fffffffu];
+k_seg0(const aff* __restrict__ pts, const u32* __restrict__ entries, const u32* __restrict__ keys,
+       const u32* __restrict__ total_p, u32 K, u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys,
+       jac* __restrict__ opart) {
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= max_chunks) return;
+    const u32 E = *total_p;
+    const u32 beg = j * K;
+    jac hv, tv;
+    jac_set_inf(hv);
+    jac_set_inf(tv);
+    u32 hk = kEmpty, tk = kEmpty;
+    if (beg < E) {
+        const u32 end = min(beg + K, E);
+        const bool before = beg > 0 && keys[beg - 1] == keys[beg];
+        const bool after = end < E && keys[end] == keys[end - 1];
+        u32 cur = keys[beg];
+        bool first = true;
+        jac acc;
+        jac_set_inf(acc);
+        u32 ent = entries[beg];
+        aff nxt = pts[ent & 0x7fffffffu];
+        for (u32 e = beg; e < end; ++e) {
+            const u32 k = keys[e];
+            aff P = nxt;
+            const bool neg = (ent >> 31) != 0;
+            if (e + 1 < end) {
+                ent = entries[e + 1];
+                nxt = pts[ent & 0x7fffffffu];
+            }
+            if (k != cur) {
+                if (first && before) { hk = cur; hv = acc; } else { bsum[cur] = acc; }
+                first = false;
+                cur = k;
+                jac_set_inf(acc);
+            }
+            if (neg) fe_neg(P.y, P.y);
+            jac_madd(acc, acc, P);
         }
-        if (neg) fe_neg(P.y, P.y);
-        jac_madd(acc, acc, P);
+        if (first && before) { hk = cur; hv = acc; }
+        else if (after) { tk = cur; tv = acc; }
+        else { bsum[cur] = acc; }
     }
-    bsum[b] = acc;
+    okeys[2 * j] = hk;
+    opart[2 * j] = hv;
+    okeys[2 * j + 1] = tk;
+    opart[2 * j + 1] = tv;
+}
+
+__global__ void __launch_bounds__(kMsmBlock)
+k_seg_flags(const u32* __restrict__ keys, u32 n, u8* __restrict__ flags) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = keys[i] != kEmpty;
+}
+
+// Higher levels: the same chunked run reduction over the compacted, still
+// bucket-ordered partials of the previous level (Jacobian additions).
+__global__ void __launch_bounds__(kMsmBlock)
+k_seg(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys,
+      const jac* __restrict__ ipart, u32 K, u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys,
+      jac* __restrict__ opart) {
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= max_chunks) return;
+    const u32 E = *count_p;
+    const u32 beg = j * K;
+    jac hv, tv;
+    jac_set_inf(hv);
+    jac_set_inf(tv);
+    u32 hk = kEmpty, tk = kEmpty;
+    if (beg < E) {
+        const u32 end = min(beg + K, E);
+        const bool before = beg > 0 && ikeys[idx[beg - 1]] == ikeys[idx[beg]];
+        const bool after = end < E && ikeys[idx[end]] == ikeys[idx[end - 1]];
+        u32 cur = ikeys[idx[beg]];
+        bool first = true;
+        jac acc;
+        jac_set_inf(acc);
+        for (u32 e = beg; e < end; ++e) {
+            const u32 s = idx[e];
+            const u32 k = ikeys[s];
+            if (k != cur) {
+                if (first && before) { hk = cur; hv = acc; } else { bsum[cur] = acc; }
+                first = false;
+                cur = k;
+                jac_set_inf(acc);
+            }
+            jac_add(acc, acc, ipart[s]);
+        }
+        if (first && before) { hk = cur; hv = acc; }
+        else if (after) { tk = cur; tv = acc; }
+        else { bsum[cur] = acc; }
+    }
+    okeys[2 * j] = hk;
+    opart[2 * j] = hv;
+    okeys[2 * j + 1] = tk;
+    opart[2 * j + 1] = tv;
 }
 
 // r = m * a for a small non-negative integer m (double-and-add).
@@ -235,18 +325,10 @@ hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33,
     return hipGetLastError();
 }
 
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, const MsmPlan& p,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(n, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
-                       cursor, entries, p.c, p.windows);
-    return hipGetLastError();
-}
-
-hipError_t launch_msm_buckets(const void* pts, const uint32_t* offsets, const uint32_t* entries, void* bsum,
+hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
                               const MsmPlan& p, hipStream_t st) {
-    const uint32_t nb = (uint32_t)p.windows * (uint32_t)p.buckets;
-    hipLaunchKernelGGL(k_msm_buckets, dim3(blocks_for(nb, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts,
-                       offsets, entries, (jac*)bsum, nb);
+    hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(n, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
+                       cursor, entries, keys, p.c, p.windows);
     return hipGetLastError();
 }
 
@@ -295,8 +377,9 @@ MsmPlan msm_plan(size_t n, int c) {
 }
 
 MsmEngine::~MsmEngine() {
-    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_,
-                           &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_};
+    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_, &keys_,
+                           &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
+                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_};
     for (DeviceBuffer* b : all) b->release();
 }
 
@@ -307,19 +390,33 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     const size_t nbk = (size_t)p.windows * p.buckets;
     const size_t nseg = (size_t)p.windows * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac);
+    const size_t max_entries = n * (size_t)p.windows;
+    const uint32_t K = kSegChunk;
+    const size_t chunks0 = (max_entries + K - 1) / K;
+    const size_t slots0 = 2 * chunks0;
     hipError_t e;
     if ((e = pts_.ensure(n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(n * sizeof(sc))) != hipSuccess ||
         (e = counts_.ensure((nbk + 1) * 4)) != hipSuccess || (e = offsets_.ensure((nbk + 1) * 4)) != hipSuccess ||
-        (e = entries_.ensure(n * p.windows * 4)) != hipSuccess || (e = bsum_.ensure(nbk * jsz)) != hipSuccess ||
-        (e = seg_.ensure(nseg * jsz)) != hipSuccess || (e = red_a_.ensure(nseg * jsz)) != hipSuccess ||
-        (e = red_b_.ensure(nseg * jsz)) != hipSuccess || (e = err_.ensure(4)) != hipSuccess)
+        (e = entries_.ensure(max_entries * 4)) != hipSuccess || (e = keys_.ensure(max_entries * 4)) != hipSuccess ||
+        (e = bsum_.ensure(nbk * jsz)) != hipSuccess || (e = seg_.ensure(nseg * jsz)) != hipSuccess ||
+        (e = red_a_.ensure(nseg * jsz)) != hipSuccess || (e = red_b_.ensure(nseg * jsz)) != hipSuccess ||
+        (e = err_.ensure(4)) != hipSuccess || (e = slot_keys_[0].ensure(slots0 * 4)) != hipSuccess ||
+        (e = slot_keys_[1].ensure(slots0 * 4)) != hipSuccess || (e = slot_part_[0].ensure(slots0 * jsz)) != hipSuccess ||
+        (e = slot_part_[1].ensure(slots0 * jsz)) != hipSuccess || (e = sel_idx_.ensure(slots0 * 4)) != hipSuccess ||
+        (e = sel_flags_.ensure(slots0)) != hipSuccess || (e = sel_count_.ensure(16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     uint32_t* counts = (uint32_t*)counts_.ptr;
     uint32_t* offsets = (uint32_t*)offsets_.ptr;
     uint32_t* err = (uint32_t*)err_.ptr;
-    size_t tmp_bytes = 0;
+    uint32_t* sel_idx = (uint32_t*)sel_idx_.ptr;
+    uint32_t* sel_count = (uint32_t*)sel_count_.ptr;
+    size_t tmp_bytes = 0, sel_bytes = 0;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, offsets, (int)(nbk + 1), st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
+                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "scan size query", e);
+    if (sel_bytes > tmp_bytes) tmp_bytes = sel_bytes;
     if ((e = scan_tmp_.ensure(tmp_bytes + 16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
 
@@ -332,11 +429,38 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket scan", e);
     e = hipMemcpyAsync(counts, offsets, (nbk + 1) * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return be_->fail(GpuError::Memory, "MSM cursor copy", e);
-    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint32_t*)entries_.ptr, p, st)) != hipSuccess ||
-        (e = launch_msm_buckets(pts_.ptr, offsets, (const uint32_t*)entries_.ptr, bsum_.ptr, p, st)) !=
-            hipSuccess ||
-        (e = launch_msm_segments(bsum_.ptr, seg_.ptr, p, st)) != hipSuccess)
-        return be_->fail(GpuError::Launch, "MSM bucket phase", e);
+    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint32_t*)entries_.ptr, (uint32_t*)keys_.ptr, p,
+                                st)) != hipSuccess)
+        return be_->fail(GpuError::Launch, "MSM scatter", e);
+    // bucket sums: empty buckets stay at infinity; level 0 over the sorted
+    // entries (total = offsets[nbk]), then chunked reductions of the compacted
+    // partials until a single chunk remains.
+    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, (jac*)bsum_.ptr,
+                       (uint32_t)nbk);
+    hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts_.ptr,
+                       (const uint32_t*)entries_.ptr, (const uint32_t*)keys_.ptr, (const uint32_t*)(offsets + nbk),
+                       K, (uint32_t)chunks0, (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
+    if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
+    size_t slots = slots0;
+    int lv = 0;
+    for (;;) {
+        const size_t chunks = (slots + K - 1) / K;
+        hipLaunchKernelGGL(k_seg_flags, dim3(blocks_for(slots, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)slot_keys_[lv].ptr, (uint32_t)slots, (uint8_t*)sel_flags_.ptr);
+        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
+                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
+        hipLaunchKernelGGL(k_seg, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count,
+                           (const uint32_t*)slot_keys_[lv].ptr, (const jac*)slot_part_[lv].ptr, K, (uint32_t)chunks,
+                           (jac*)bsum_.ptr, (uint32_t*)slot_keys_[lv ^ 1].ptr, (jac*)slot_part_[lv ^ 1].ptr);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
+        lv ^= 1;
+        if (chunks == 1) break;   // one chunk held every remaining partial: all buckets complete
+        slots = 2 * chunks;
+    }
+    if ((e = launch_msm_segments(bsum_.ptr, seg_.ptr, p, st)) != hipSuccess)
+        return be_->fail(GpuError::Launch, "MSM window sums", e);
     // tree-reduce segments per window until one point per window remains
     uint32_t len = (uint32_t)(p.buckets / p.seg_len);
     const void* cur = seg_.ptr;

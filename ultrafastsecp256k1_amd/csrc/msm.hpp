@@ -10,7 +10,10 @@
 //   2. signed c-bit digits (W = ceil(257/c) windows, |d| <= 2^(c-1));
 //      histogram of (window, |d|) bucket ids, exclusive scan, scatter of
 //      (term index | sign) into bucket-sorted order;
-//   3. bucket accumulation: one lane per bucket, mixed additions;
+//   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
+//      entries (mixed additions), with chunk-boundary partials merged in a
+//      few compaction + reduction levels, so no lane adds more than 32 terms
+//      even when one bucket holds half of all terms;
 //   4. per-window weighted sum sum_b (b+1) B_b by segments (running sums plus
 //      one small scalar multiplication per segment) and a block tree;
 //   5. Horner combination of the windows, affine normalisation, compression.
@@ -39,9 +42,7 @@ MsmPlan msm_plan(size_t n, int c);
 // Device launchers (msm.hip)
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
                               uint32_t* counts, const MsmPlan& p, uint32_t* err, hipStream_t st);
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, const MsmPlan& p,
-                              hipStream_t st);
-hipError_t launch_msm_buckets(const void* pts, const uint32_t* offsets, const uint32_t* entries, void* bsum,
+hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
                               const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
@@ -64,8 +65,8 @@ private:
     GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                      int32_t* d_status, int c, hipStream_t st);
     RocmBackend* be_;
-    DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, bsum_, seg_, red_a_, red_b_, err_,
-        scan_tmp_, out_;
+    DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, keys_, bsum_, seg_, red_a_, red_b_, err_,
+        scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_flags_, sel_count_;
 };
 
 }  // namespace ufs

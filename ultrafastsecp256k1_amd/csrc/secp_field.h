@@ -71,60 +71,154 @@ UFS_DEV u32 subc(u32 a, u32 b, u32 bin, u32& bout) {
 // acc(96) += a*b, with acc = hi:lo64.  One v_mad_u64_u32 (carry-out to an SGPR
 // pair) plus one v_addc_co_u32 into the hi word.
 UFS_DEV void mac96(u64& lo, u32& hi, u32 a, u32 b) {
-#if defined(__HIP_DEVICE_COMPILE__)
     u64 cc, dead;
     asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(lo), "=s"(cc) : "v"(a), "v"(b));
     asm("v_addc_co_u32 %0, %1, %0, 0, %2" : "+v"(hi), "=s"(dead) : "s"(cc));
-#else
-    u64 p = (u64)a * b;
-    lo += p;
-    hi += (lo < p);
-#endif
 }
 
-// Full 256x256 -> 512-bit product, t[0..15].
+// Column multiply-accumulate blocks (generated): (hi:lo) = lo + sum a_k*b_k, hi = carries.
+UFS_DEV void col1(u64& lo, u32& hi, u32 a0, u32 b0) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\ts_nop 0\n\tv_addc_co_u32 %1, %2, 0, 0, %2"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0));
+}
+UFS_DEV void col2(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_addc_co_u32 %1, %3, %1, 0, %3"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1));
+}
+UFS_DEV void col3(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_addc_co_u32 %1, %2, %1, 0, %2"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2));
+}
+UFS_DEV void col4(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2, u32 a3, u32 b3) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %10, %11, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_addc_co_u32 %1, %3, %1, 0, %3"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3));
+}
+UFS_DEV void col5(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2, u32 a3, u32 b3, u32 a4, u32 b4) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %10, %11, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %12, %13, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_addc_co_u32 %1, %2, %1, 0, %2"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3), "v"(a4), "v"(b4));
+}
+UFS_DEV void col6(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2, u32 a3, u32 b3, u32 a4, u32 b4, u32 a5, u32 b5) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %10, %11, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %12, %13, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %14, %15, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_addc_co_u32 %1, %3, %1, 0, %3"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3), "v"(a4), "v"(b4), "v"(a5), "v"(b5));
+}
+UFS_DEV void col7(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2, u32 a3, u32 b3, u32 a4, u32 b4, u32 a5, u32 b5, u32 a6, u32 b6) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %10, %11, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %12, %13, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %14, %15, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %16, %17, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_addc_co_u32 %1, %2, %1, 0, %2"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3), "v"(a4), "v"(b4), "v"(a5), "v"(b5), "v"(a6), "v"(b6));
+}
+UFS_DEV void col8(u64& lo, u32& hi, u32 a0, u32 b0, u32 a1, u32 b1, u32 a2, u32 b2, u32 a3, u32 b3, u32 a4, u32 b4, u32 a5, u32 b5, u32 a6, u32 b6, u32 a7, u32 b7) {
+    u64 sa, sb;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\tv_mad_u64_u32 %0, %3, %6, %7, %0\n\tv_addc_co_u32 %1, %2, 0, 0, %2\n\tv_mad_u64_u32 %0, %2, %8, %9, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %10, %11, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %12, %13, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %14, %15, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_mad_u64_u32 %0, %2, %16, %17, %0\n\tv_addc_co_u32 %1, %3, %1, 0, %3\n\tv_mad_u64_u32 %0, %3, %18, %19, %0\n\tv_addc_co_u32 %1, %2, %1, 0, %2\n\tv_addc_co_u32 %1, %3, %1, 0, %3"
+        : "+v"(lo), "=&v"(hi), "=&s"(sa), "=&s"(sb)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3), "v"(a4), "v"(b4), "v"(a5), "v"(b5), "v"(a6), "v"(b6), "v"(a7), "v"(b7));
+}
+
+
+// Full 256x256 -> 512-bit product, t[0..15], by columns (product scanning).
+// Each column is one asm block of v_mad_u64_u32 / v_addc_co_u32 pairs whose
+// carry flags alternate between two SGPR pairs, so every carry read is one
+// instruction away from its write (no hazard wait states).  A single chain per
+// column measured faster than two interleaved column chains: at 3 waves per
+// SIMD the 10-cycle dependent-issue latency is already hidden and the extra
+// carry merging costs more than the added ILP.
 UFS_DEV void mul_512(u32 t[16], const u32 a[8], const u32 b[8]) {
     u64 lo = 0;
-    u32 hi = 0;
-#pragma unroll
-    for (int k = 0; k < 15; ++k) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int j = k - i;
-            if (j < 0 || j > 7) continue;
-            mac96(lo, hi, a[i], b[j]);
-        }
-        t[k] = (u32)lo;
-        lo = (lo >> 32) | ((u64)hi << 32);
-        hi = 0;
-    }
-    t[15] = (u32)lo;
+    u32 hi;
+    col1(lo, hi, a[0], b[0]);
+    t[0] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[0], b[1], a[1], b[0]);
+    t[1] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[0], b[2], a[1], b[1], a[2], b[0]);
+    t[2] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col4(lo, hi, a[0], b[3], a[1], b[2], a[2], b[1], a[3], b[0]);
+    t[3] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col5(lo, hi, a[0], b[4], a[1], b[3], a[2], b[2], a[3], b[1], a[4], b[0]);
+    t[4] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col6(lo, hi, a[0], b[5], a[1], b[4], a[2], b[3], a[3], b[2], a[4], b[1], a[5], b[0]);
+    t[5] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col7(lo, hi, a[0], b[6], a[1], b[5], a[2], b[4], a[3], b[3], a[4], b[2], a[5], b[1], a[6], b[0]);
+    t[6] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col8(lo, hi, a[0], b[7], a[1], b[6], a[2], b[5], a[3], b[4], a[4], b[3], a[5], b[2], a[6], b[1], a[7], b[0]);
+    t[7] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col7(lo, hi, a[1], b[7], a[2], b[6], a[3], b[5], a[4], b[4], a[5], b[3], a[6], b[2], a[7], b[1]);
+    t[8] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col6(lo, hi, a[2], b[7], a[3], b[6], a[4], b[5], a[5], b[4], a[6], b[3], a[7], b[2]);
+    t[9] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col5(lo, hi, a[3], b[7], a[4], b[6], a[5], b[5], a[6], b[4], a[7], b[3]);
+    t[10] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col4(lo, hi, a[4], b[7], a[5], b[6], a[6], b[5], a[7], b[4]);
+    t[11] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[5], b[7], a[6], b[6], a[7], b[5]);
+    t[12] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[6], b[7], a[7], b[6]);
+    t[13] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col1(lo, hi, a[7], b[7]);
+    t[14] = (u32)lo;
+    t[15] = (u32)(lo >> 32);   // hi is 0: the full product is < 2^512
 }
 
-// 256-bit square -> 512 bits.  Off-diagonal products are accumulated once per
-// column and doubled; diagonal terms are added afterwards.
+// 256-bit square -> 512 bits: the 28 off-diagonal products once (column
+// blocks as in mul_512), doubled with funnel shifts, plus the 8 squares.
 UFS_DEV void sqr_512(u32 t[16], const u32 a[8]) {
-    u64 carry = 0;   // carry into the current column (< 2^36)
+    u32 o[16];
+    u64 lo = 0;
+    u32 hi;
+    o[0] = 0;
+    col1(lo, hi, a[0], a[1]);
+    o[1] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col1(lo, hi, a[0], a[2]);
+    o[2] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[0], a[3], a[1], a[2]);
+    o[3] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[0], a[4], a[1], a[3]);
+    o[4] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[0], a[5], a[1], a[4], a[2], a[3]);
+    o[5] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[0], a[6], a[1], a[5], a[2], a[4]);
+    o[6] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col4(lo, hi, a[0], a[7], a[1], a[6], a[2], a[5], a[3], a[4]);
+    o[7] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[1], a[7], a[2], a[6], a[3], a[5]);
+    o[8] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col3(lo, hi, a[2], a[7], a[3], a[6], a[4], a[5]);
+    o[9] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[3], a[7], a[4], a[6]);
+    o[10] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col2(lo, hi, a[4], a[7], a[5], a[6]);
+    o[11] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col1(lo, hi, a[5], a[7]);
+    o[12] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    col1(lo, hi, a[6], a[7]);
+    o[13] = (u32)lo; lo = (lo >> 32) | ((u64)hi << 32);
+    o[14] = (u32)lo;
+    o[15] = (u32)(lo >> 32);
+    // t = 2*o + sum a_i^2 * 2^(64 i)
+    u32 d[16];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) {
-        u64 olo = 0;
-        u32 ohi = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int j = k - i;
-            if (j <= i || j > 7) continue;
-            mac96(olo, ohi, a[i], a[j]);
-        }
-        // acc = 2*off + carry (+ diag)
-        u32 dhi = (ohi << 1) | (u32)(olo >> 63);
-        u64 dlo = olo << 1;
-        u64 lo = dlo + carry;
-        u32 hi = dhi + (lo < dlo);
-        if ((k & 1) == 0) mac96(lo, hi, a[k >> 1], a[k >> 1]);
-        t[k] = (u32)lo;
-        carry = (lo >> 32) | ((u64)hi << 32);
+    for (int i = 0; i < 8; ++i) {
+        const u64 s = (u64)a[i] * a[i];
+        d[2 * i] = (u32)s;
+        d[2 * i + 1] = (u32)(s >> 32);
     }
-    t[15] = (u32)carry;
+    u32 c = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const u32 dbl = (k == 0) ? (o[0] << 1) : __builtin_amdgcn_alignbit(o[k], o[k - 1], 31);
+        t[k] = addc(dbl, d[k], c, c);
+    }
 }
 
 // ---------------------------------------------------------------------------

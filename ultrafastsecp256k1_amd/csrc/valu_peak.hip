@@ -63,6 +63,44 @@ __global__ void __launch_bounds__(256) probe(u32* out, int iters, u32 seed) {
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Latency probe: ONE dependent v_mad_u64_u32 chain per lane, one wave per
+// SIMD; cycles per step = dependent-issue latency.
+__global__ void __launch_bounds__(64) probe_latency(u32* out, int iters, u32 seed) {
+    u32 a = seed ^ threadIdx.x, b = seed * 3u + blockIdx.x;
+    u64 acc = (u64)a * 0x9E3779B9u;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            u64 cc;
+            asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc), "=s"(cc) : "v"(a), "v"(b));
+        }
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = (u32)acc ^ (u32)(acc >> 32);
+}
+
+static void run_latency(int iters, int cus, double clk_ghz) {
+    u32* out;
+    const int blocks = cus * 4;   // one 64-lane wave per SIMD
+    CHK(hipMalloc(&out, (size_t)blocks * 64 * 4));
+    hipEvent_t e0, e1;
+    CHK(hipEventCreate(&e0));
+    CHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(probe_latency, dim3(blocks), dim3(64), 0, 0, out, 8, 1u);
+    CHK(hipDeviceSynchronize());
+    CHK(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(probe_latency, dim3(blocks), dim3(64), 0, 0, out, iters, 1u);
+    CHK(hipEventRecord(e1, 0));
+    CHK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHK(hipEventElapsedTime(&ms, e0, e1));
+    const double steps = (double)iters * 32.0;
+    std::printf("{\"inst\": \"v_mad_u64_u32_dependent_chain\", \"ms\": %.3f, \"cycles_per_step\": %.2f}\n", ms,
+                ms * 1e-3 * clk_ghz * 1e9 / steps);
+    CHK(hipFree(out));
+    CHK(hipEventDestroy(e0));
+    CHK(hipEventDestroy(e1));
+}
+
 template <int OP>
 static void run(const char* name, int iters, int blocks, double clk_ghz, int cus) {
     u32* out;
@@ -103,5 +141,6 @@ int main(int argc, char** argv) {
     run<4>("v_lshl_add_u64", iters, blocks, clk, cus);
     run<5>("v_add_u32", iters, blocks, clk, cus);
     run<6>("v_mad_u32_u24", iters, blocks, clk, cus);
+    run_latency(iters, cus, clk);
     return 0;
 }

@@ -1,0 +1,103 @@
+"""Host-side checks of the C-ABI library that need no GPU.
+
+* the shared library loads and exports every function declared with
+  UFSECP_API in include/ufsecp/*.h;
+* discovery calls behave as the reference's (include/ufsecp/ufsecp_gpu.h:109-135,
+  src/gpu/src/gpu_registry.cpp:65-77): ids other than ROCm are unavailable,
+  backend 0 cannot create a context, NULL arguments give NULL_ARG;
+* the workload generator's host logic (plan statistics, corruption schedule).
+"""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+import ultrafastsecp256k1_amd as ufs
+from ultrafastsecp256k1_amd import workload as wl
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in sorted((ROOT / "include" / "ufsecp").glob("*.h")):
+        text = h.read_text()
+        for m in re.finditer(r"UFSECP_API\s+[^;{]*?\b(ufsecp_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_symbols()
+    for ref_name in ("ufsecp_gpu_backend_count", "ufsecp_gpu_ctx_create", "ufsecp_gpu_ecdsa_verify_batch",
+                     "ufsecp_gpu_schnorr_verify_batch", "ufsecp_gpu_msm", "ufsecp_gpu_ecdsa_verify_collect",
+                     "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_error_str"):
+        assert ref_name in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ufs.load_library()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+    for n in ufs.ABI_SYMBOLS:
+        assert hasattr(lib, n), n
+
+
+def test_discovery_without_context():
+    lib = ufs.load_library()
+    lib.ufsecp_gpu_backend_name.restype = ctypes.c_char_p
+    assert lib.ufsecp_gpu_backend_name(4) == b"ROCm"
+    assert lib.ufsecp_gpu_backend_name(1) == b"CUDA"
+    assert lib.ufsecp_gpu_backend_name(2) == b"OpenCL"
+    assert lib.ufsecp_gpu_backend_name(3) == b"Metal"
+    for bid in (0, 1, 2, 3, 99):
+        assert lib.ufsecp_gpu_is_available(bid) == 0
+        assert lib.ufsecp_gpu_device_count(bid) == 0
+    ids = (ctypes.c_uint32 * 4)()
+    assert lib.ufsecp_gpu_backend_count(ids, 4) == 1
+    assert ids[0] == 4
+    assert lib.ufsecp_gpu_backend_count(None, 0) == 1
+    ctx = ctypes.c_void_p(1)
+    assert lib.ufsecp_gpu_ctx_create(ctypes.byref(ctx), 0, 0) == 100  # GPU_UNAVAILABLE
+    assert ctx.value is None
+    assert lib.ufsecp_gpu_ctx_create(None, 4, 0) == 1  # NULL_ARG
+    lib.ufsecp_gpu_error_str.restype = ctypes.c_char_p
+    assert lib.ufsecp_gpu_error_str(0) == b"OK"
+    assert lib.ufsecp_gpu_error_str(103)
+
+
+def test_null_context_calls():
+    lib = ufs.load_library()
+    out = (ctypes.c_uint8 * 4)(7, 7, 7, 7)
+    buf = (ctypes.c_uint8 * 256)()
+    assert lib.ufsecp_gpu_ecdsa_verify_batch(None, buf, buf, buf, ctypes.c_size_t(4), out) == 1
+    assert lib.ufsecp_gpu_schnorr_verify_batch(None, buf, buf, buf, ctypes.c_size_t(4), out) == 1
+    assert lib.ufsecp_gpu_msm(None, buf, buf, ctypes.c_size_t(1), out) == 1
+    assert lib.ufsecp_gpu_is_ready(None) == 0
+    assert lib.ufsecp_gpu_last_error(None) == 1
+
+
+def test_mix_plan_statistics():
+    plan = wl.mix_plan(10_000, 4096, wl.SEED_MIX)
+    kinds = np.array([k for k, _ in plan])
+    sizes = np.array([s for _, s in plan])
+    assert len(plan) == 10_000
+    assert sizes.min() >= 1 and sizes.max() <= 4096
+    # log-uniform on [1, 4096]: mean (4096 - 1) / ln(4096) ~ 492
+    assert 440 < sizes.mean() < 545
+    assert 0.67 < (kinds == ufs.KIND_ECDSA).mean() < 0.73
+    # the plan is a pure function of its seed
+    assert wl.mix_plan(10_000, 4096, wl.SEED_MIX) == plan
+
+
+def test_corruption_schedule_rate():
+    b = wl.generate("schnorr", 20_000, wl.SEED_SCHNORR)
+    rate = b.corrupted.astype(float).mean()
+    assert 0.005 < rate < 0.015
+    # item i of a batch equals item 0 of the batch started at i
+    tail = wl.generate("schnorr", 4, wl.SEED_SCHNORR, first=777)
+    assert np.array_equal(tail.sigs, b.sigs[777:781])
+    assert np.array_equal(tail.pubkeys, b.pubkeys[777:781])
+    assert np.array_equal(tail.corrupted, b.corrupted[777:781])

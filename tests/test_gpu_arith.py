@@ -14,7 +14,8 @@ from ec_math import BETA, G, LAMBDA, N, P, add, from_limbs, lift_x, mul, to_limb
 pytestmark = pytest.mark.gpu
 
 OP_FE_MUL, OP_FE_SQR, OP_FE_ADD, OP_FE_SUB, OP_FE_INV, OP_FE_SQRT = 0, 1, 2, 3, 4, 5
-OP_SC_MUL, OP_SC_INV, OP_GLV = 10, 11, 12
+OP_FE_INV_FERMAT = 8
+OP_SC_MUL, OP_SC_INV, OP_GLV, OP_SC_INV_FERMAT = 10, 11, 12, 13
 OP_GEN_MUL, OP_VAR_MUL = 20, 21
 
 M256 = (1 << 256) - 1
@@ -70,6 +71,39 @@ def test_scalar_mul_inv(gpu):
     out = gpu.selftest(OP_SC_INV, _pack(nz), _pack(nz))
     for i, x in enumerate(nz):
         assert from_limbs(out[i, :8]) == pow(x, -1, N)
+
+
+def _inv_inputs(m, n, seed):
+    """Random residues plus inputs that stress the divsteps batches: 0, 1,
+    m-1, powers of two (long runs of even steps), 2^k - 1 (long odd runs),
+    (m+1)/2 and values that differ from m in the low word only."""
+    rnd = random.Random(seed)
+    xs = [0, 1, 2, 3, m - 1, m - 2, (m + 1) // 2, (m - 1) // 2, m - (1 << 32), (1 << 256) - 1 - m]
+    xs += [1 << k for k in range(0, 256, 7)] + [(1 << k) - 1 for k in range(2, 256, 9)]
+    xs += [m - (1 << k) for k in range(1, 250, 13)]
+    xs += [rnd.randrange(m) for _ in range(n)]
+    return [x % m for x in xs]
+
+
+@pytest.mark.parametrize("mod", ["p", "n"])
+def test_divsteps_inverse(gpu, mod):
+    """Divsteps inversion == Python pow(x, -1, m) == the a^(m-2) chain, 0 -> 0."""
+    m = P if mod == "p" else N
+    op, op_fermat = (OP_FE_INV, OP_FE_INV_FERMAT) if mod == "p" else (OP_SC_INV, OP_SC_INV_FERMAT)
+    xs = _inv_inputs(m, 4000, 11 if mod == "p" else 12)
+    a = _pack(xs)
+    out = gpu.selftest(op, a, a)
+    ref = gpu.selftest(op_fermat, a, a)
+    for i, x in enumerate(xs):
+        exp = pow(x, -1, m) if x else 0
+        assert from_limbs(out[i, :8]) == exp, hex(x)
+        assert from_limbs(ref[i, :8]) == exp, hex(x)
+    if mod == "p":   # weakly reduced inputs in [p, 2^256)
+        ys = [P, P + 1, P + 977, (1 << 256) - 1]
+        out = gpu.selftest(OP_FE_INV, _pack(ys), _pack(ys))
+        for i, y in enumerate(ys):
+            exp = pow(y % P, -1, P) if y % P else 0
+            assert from_limbs(out[i, :8]) == exp, hex(y)
 
 
 def test_glv_split(gpu):

@@ -89,6 +89,8 @@ RocmBackend::~RocmBackend() {
     d_descs_.release();
     d_tiles_.release();
     d_words_.release();
+    d_slots_.release();
+    if (verify_done_) (void)hipEventDestroy(verify_done_);
     h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
@@ -129,6 +131,22 @@ GpuError RocmBackend::init(uint32_t device_index) {
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(midstate)", e);
     e = launch_bip340_midstate(d_mid_, stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "BIP-340 midstate", e);
+
+    // Persistent verify grids (one resident wave set) and their lane slots.
+    hipDeviceProp_t p;
+    if ((e = hipGetDeviceProperties(&p, device_)) != hipSuccess) return fail(GpuError::Device, "properties", e);
+    cus_ = p.multiProcessorCount;
+    unsigned max_grid = 0;
+    for (int k = 0; k < 3; ++k) {
+        verify_grid_[k] = (unsigned)(cus_ * verify_blocks_per_cu(k));
+        if (verify_grid_[k] > max_grid) max_grid = verify_grid_[k];
+    }
+    if ((e = d_slots_.ensure((size_t)max_grid * kVerifyBlock * kSlotBytes)) != hipSuccess)
+        return fail(GpuError::Memory, "hipMalloc(lane slots)", e);
+    if ((e = d_words_.ensure(64 * sizeof(uint32_t))) != hipSuccess)
+        return fail(GpuError::Memory, "hipMalloc(work counter)", e);
+    if ((e = hipEventCreateWithFlags(&verify_done_, hipEventDisableTiming)) != hipSuccess)
+        return fail(GpuError::Backend, "hipEventCreate", e);
     e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return fail(GpuError::Backend, "init synchronize", e);
     ready_ = true;
@@ -140,6 +158,34 @@ GpuError RocmBackend::synchronize() {
     hipError_t e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return fail(GpuError::Backend, "hipStreamSynchronize", e);
     return GpuError::Ok;
+}
+
+hipError_t RocmBackend::verify_begin(hipStream_t st) {
+    if (verify_stream_ && verify_stream_ != st) return hipStreamWaitEvent(st, verify_done_, 0);
+    return hipSuccess;
+}
+
+hipError_t RocmBackend::verify_end(hipStream_t st) {
+    verify_stream_ = st;
+    return hipEventRecord(verify_done_, st);
+}
+
+hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStream_t st) {
+    if (d.count == 0) return hipSuccess;
+    const uint32_t tiles = (d.count + kTileItems - 1) / kTileItems;
+    const int kind = d.kind == 0 ? 0 : 1;
+    const unsigned waves_per_block = kVerifyBlock / 64;
+    unsigned grid = (tiles + waves_per_block - 1) / waves_per_block;
+    if (grid > verify_grid_[kind]) grid = verify_grid_[kind];
+    VerifyLaunch a{};
+    a.single = d;
+    a.slots = d_slots_.ptr;
+    a.single_tiles = tiles;
+    a.collect = collect ? 1u : 0u;
+    hipError_t e = verify_begin(st);
+    if (e == hipSuccess) e = launch_verify(kind, a, gdesc_, d_mid_, grid, st);
+    if (e == hipSuccess) e = verify_end(st);
+    return e;
 }
 
 GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
@@ -162,10 +208,7 @@ GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8
     const uint8_t* dp = (const uint8_t*)d_in_[1].ptr;
     const uint8_t* ds = (const uint8_t*)d_in_[2].ptr;
     uint8_t* dout = (uint8_t*)d_out_.ptr;
-    if (schnorr)
-        e = launch_schnorr_verify(dm, dp, ds, (uint32_t)count, gdesc_, d_mid_, dout, collect, stream_);
-    else
-        e = launch_ecdsa_verify(dm, dp, ds, (uint32_t)count, gdesc_, dout, collect, stream_);
+    e = launch_single(BatchDesc{schnorr ? 1u : 0u, (uint32_t)count, dm, dp, ds, dout}, collect, stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "verify kernel launch", e);
     e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
     if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
@@ -188,8 +231,8 @@ GpuError RocmBackend::schnorr_verify_batch(const uint8_t* msgs, const uint8_t* p
 GpuError RocmBackend::ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,
                                           size_t count, uint8_t* d_out, hipStream_t st) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
-    hipError_t e = launch_ecdsa_verify(d_msgs, d_pubs33, d_sigs, (uint32_t)count, gdesc_, d_out, false,
-                                       st ? st : stream_);
+    hipError_t e = launch_single(BatchDesc{0u, (uint32_t)count, d_msgs, d_pubs33, d_sigs, d_out}, false,
+                                 st ? st : stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "ECDSA verify launch", e);
     clear_error();
     return GpuError::Ok;
@@ -198,8 +241,8 @@ GpuError RocmBackend::ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* 
 GpuError RocmBackend::schnorr_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubx, const uint8_t* d_sigs,
                                             size_t count, uint8_t* d_out, hipStream_t st) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
-    hipError_t e = launch_schnorr_verify(d_msgs, d_pubx, d_sigs, (uint32_t)count, gdesc_, d_mid_, d_out, false,
-                                         st ? st : stream_);
+    hipError_t e = launch_single(BatchDesc{1u, (uint32_t)count, d_msgs, d_pubx, d_sigs, d_out}, false,
+                                 st ? st : stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "Schnorr verify launch", e);
     clear_error();
     return GpuError::Ok;
@@ -210,10 +253,8 @@ GpuError RocmBackend::verify_each_device(const BatchDesc* batches, size_t nbatch
     st = st ? st : stream_;
     for (size_t b = 0; b < nbatches; ++b) {
         const BatchDesc& d = batches[b];
-        if (d.count == 0) continue;
-        hipError_t e = d.kind == 0
-                           ? launch_ecdsa_verify(d.msgs, d.pubs, d.sigs, d.count, gdesc_, d.out, false, st)
-                           : launch_schnorr_verify(d.msgs, d.pubs, d.sigs, d.count, gdesc_, d_mid_, d.out, false, st);
+        if (d.kind > 1) return fail(GpuError::BadInput, "batch kind must be 0 (ECDSA) or 1 (BIP-340)");
+        hipError_t e = launch_single(d, false, st);
         if (e != hipSuccess) return fail(GpuError::Launch, "per-batch verify launch", e);
     }
     clear_error();
@@ -238,8 +279,7 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
     const size_t desc_bytes = nbatches * sizeof(BatchDesc), tile_bytes = ntiles * sizeof(TileDesc);
     hipError_t e;
     if ((e = h_tables_.ensure(desc_bytes + tile_bytes + 64)) != hipSuccess ||
-        (e = d_descs_.ensure(desc_bytes)) != hipSuccess || (e = d_tiles_.ensure(tile_bytes)) != hipSuccess ||
-        (e = d_words_.ensure(64)) != hipSuccess)
+        (e = d_descs_.ensure(desc_bytes)) != hipSuccess || (e = d_tiles_.ensure(tile_bytes)) != hipSuccess)
         return fail(GpuError::Memory, "multi-batch tables", e);
     // The pinned staging area may still feed an earlier copy on this stream.
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(GpuError::Backend, "staging wait", e);
@@ -256,18 +296,20 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
         (e = hipMemcpyAsync(d_tiles_.ptr, tl, tile_bytes, hipMemcpyHostToDevice, st)) != hipSuccess ||
         (e = hipMemcpyAsync(d_words, words, 4, hipMemcpyHostToDevice, st)) != hipSuccess)
         return fail(GpuError::Memory, "copy multi-batch tables", e);
-    if (multi_grid_ == 0) {
-        hipDeviceProp_t p;
-        if ((e = hipGetDeviceProperties(&p, device_)) != hipSuccess) return fail(GpuError::Device, "properties", e);
-        cus_ = p.multiProcessorCount;
-        multi_grid_ = (unsigned)(cus_ * verify_multi_blocks_per_cu());
-    }
     const BatchDesc* dd = (const BatchDesc*)d_descs_.ptr;
     const TileDesc* dt = (const TileDesc*)d_tiles_.ptr;
+    VerifyLaunch a{};
+    a.descs = dd;
+    a.tiles = dt;
+    a.ntiles_dev = d_words;
+    a.counter = d_words + 16;
+    a.slots = d_slots_.ptr;
+    if ((e = verify_begin(st)) != hipSuccess) return fail(GpuError::Backend, "verify stream order", e);
     if (!use_graph) {
         if ((e = launch_u32_fill(d_words + 16, 0u, 1, st)) != hipSuccess ||
-            (e = launch_verify_multi(dd, dt, d_words, d_words + 16, gdesc_, d_mid_, multi_grid_, st)) != hipSuccess)
+            (e = launch_verify(2, a, gdesc_, d_mid_, verify_grid_[2], st)) != hipSuccess)
             return fail(GpuError::Launch, "multi-batch verify launch", e);
+        if ((e = verify_end(st)) != hipSuccess) return fail(GpuError::Backend, "verify event", e);
         clear_error();
         return GpuError::Ok;
     }
@@ -283,7 +325,7 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
         if ((e = hipStreamBeginCapture(graph_stream_, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return fail(GpuError::Backend, "hipStreamBeginCapture", e);
         hipError_t e1 = launch_u32_fill(d_words + 16, 0u, 1, graph_stream_);
-        hipError_t e2 = launch_verify_multi(dd, dt, d_words, d_words + 16, gdesc_, d_mid_, multi_grid_, graph_stream_);
+        hipError_t e2 = launch_verify(2, a, gdesc_, d_mid_, verify_grid_[2], graph_stream_);
         e = hipStreamEndCapture(graph_stream_, &graph_);
         if (e1 != hipSuccess || e2 != hipSuccess || e != hipSuccess)
             return fail(GpuError::Backend, "graph capture", e != hipSuccess ? e : (e1 != hipSuccess ? e1 : e2));
@@ -293,27 +335,28 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
         graph_tiles_ = dt;
     }
     if ((e = hipGraphLaunch(graph_exec_, st)) != hipSuccess) return fail(GpuError::Launch, "hipGraphLaunch", e);
+    if ((e = verify_end(st)) != hipSuccess) return fail(GpuError::Backend, "verify event", e);
     clear_error();
     return GpuError::Ok;
 }
 
 GpuError RocmBackend::selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t count) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
-    DeviceBuffer da, db, dout;
+    DeviceBuffer da, db, dout, dslot;
     hipError_t e;
     if ((e = da.ensure(count * 64)) != hipSuccess || (e = db.ensure(count * 64)) != hipSuccess ||
-        (e = dout.ensure(count * 128)) != hipSuccess) {
-        da.release(); db.release(); dout.release();
+        (e = dout.ensure(count * 128)) != hipSuccess || (e = dslot.ensure(count * kSlotBytes)) != hipSuccess) {
+        da.release(); db.release(); dout.release(); dslot.release();
         return fail(GpuError::Memory, "selftest buffers", e);
     }
     e = hipMemcpyAsync(da.ptr, a, count * 64, hipMemcpyHostToDevice, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(db.ptr, b, count * 64, hipMemcpyHostToDevice, stream_);
     if (e == hipSuccess)
         e = launch_selftest(op, (const uint32_t*)da.ptr, (const uint32_t*)db.ptr, (uint32_t*)dout.ptr,
-                            (uint32_t)count, gdesc_, stream_);
+                            (uint32_t)count, gdesc_, dslot.ptr, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout.ptr, count * 128, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    da.release(); db.release(); dout.release();
+    da.release(); db.release(); dout.release(); dslot.release();
     if (e != hipSuccess) return fail(GpuError::Backend, "selftest", e);
     clear_error();
     return GpuError::Ok;

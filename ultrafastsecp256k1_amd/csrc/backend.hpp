@@ -106,6 +106,12 @@ public:
 private:
     GpuError verify_host(bool schnorr, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
                          size_t count, uint8_t* out, bool collect);
+    // One batch through the persistent kernel (single mode).
+    hipError_t launch_single(const BatchDesc& d, bool collect, hipStream_t st);
+    // Verify launches share the lane slots and the work counter: a launch on
+    // a stream other than the previous one waits for the previous launch.
+    hipError_t verify_begin(hipStream_t st);
+    hipError_t verify_end(hipStream_t st);
 
     int device_ = -1;
     bool ready_ = false;
@@ -124,7 +130,10 @@ private:
     hipStream_t graph_stream_ = nullptr;
     const void* graph_descs_ = nullptr;
     const void* graph_tiles_ = nullptr;
-    unsigned multi_grid_ = 0;
+    unsigned verify_grid_[3] = {0, 0, 0};   // blocks: ECDSA, BIP-340, mixed
+    DeviceBuffer d_slots_;                   // kSlotBytes per lane of the largest grid
+    hipEvent_t verify_done_ = nullptr;
+    hipStream_t verify_stream_ = nullptr;
     int cus_ = 0;
     GpuError last_err_ = GpuError::Ok;
     std::string last_msg_ = "OK";

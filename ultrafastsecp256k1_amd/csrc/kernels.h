@@ -30,18 +30,15 @@ enum SelftestOp : int {
     kOpFeSqrt = 5,
     kOpFeMulChain = 6,
     kOpFeSqrChain = 7,
+    kOpFeInvFermat = 8,
     kOpScMul = 10,
     kOpScInv = 11,
     kOpGlv = 12,
+    kOpScInvFermat = 13,
     kOpGenMul = 20,
     kOpVarMul = 21,
 };
 
-hipError_t launch_ecdsa_verify(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs, uint32_t count,
-                               const GcombDesc& g, uint8_t* out, bool collect, hipStream_t st);
-hipError_t launch_schnorr_verify(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, uint32_t count,
-                                 const GcombDesc& g, const uint32_t* mid, uint8_t* out, bool collect,
-                                 hipStream_t st);
 // One verify batch of a multi-batch launch (device pointers; strides as in
 // the single-batch ABI: msgs 32, pubkeys 33 (ECDSA) or 32 (BIP-340), sigs 64).
 struct BatchDesc {
@@ -61,16 +58,34 @@ struct TileDesc {
 
 constexpr uint32_t kTileItems = 64;
 
-// ntiles and counter are device words so a captured graph stays valid when
-// the tile list changes between replays.
-hipError_t launch_verify_multi(const BatchDesc* descs, const TileDesc* tiles, const uint32_t* ntiles,
-                               uint32_t* counter, const GcombDesc& g, const uint32_t* mid, unsigned grid_blocks,
-                               hipStream_t st);
-int verify_multi_blocks_per_cu();
+// Bytes of lane-private table space the verify kernel needs per grid lane
+// (twelve 64-byte affine entries).
+constexpr size_t kSlotBytes = 768;
+
+// Arguments of the persistent verify kernel.  Single mode (tiles == nullptr):
+// the tiles of `single`, single_tiles of them.  List mode: tiles[0 .. *ntiles_dev)
+// over descs, distributed through `counter`, which must read zero at launch
+// (ntiles_dev and counter are device words so a captured graph stays valid
+// when the tile list changes between replays).
+struct VerifyLaunch {
+    BatchDesc single;
+    const BatchDesc* descs;
+    const TileDesc* tiles;
+    const uint32_t* ntiles_dev;
+    uint32_t* counter;
+    void* slots;                // kSlotBytes per lane of the grid
+    uint32_t single_tiles;
+    uint32_t collect;           // 1: valid -> zero the caller's byte, else keep
+};
+
+// kind: 0 = ECDSA, 1 = BIP-340, 2 = per batch (list mode).
+hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, const uint32_t* mid, unsigned grid,
+                         hipStream_t st);
+int verify_blocks_per_cu(int kind);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,
-                           const GcombDesc& g, hipStream_t st);
+                           const GcombDesc& g, void* slots, hipStream_t st);
 
 }  // namespace ufs

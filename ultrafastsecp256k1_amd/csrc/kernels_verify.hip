@@ -14,6 +14,10 @@
 #include "secp_ecmult.h"
 #include "secp_hash.h"
 
+#ifndef UFS_VERIFY_MIN_BLOCKS
+#define UFS_VERIFY_MIN_BLOCKS 3
+#endif
+
 namespace ufs {
 
 UFS_DEV void load_be32_aligned(u32 r[8], const u8* p) {
@@ -67,13 +71,20 @@ UFS_DEV bool ecdsa_xcheck(const jac& R, const u32 r[8]) {
     return ok;
 }
 
-// One ECDSA verification; returns the verdict.
-UFS_DEV bool ecdsa_verify_one(const u32 z_raw[8], const u32 r_raw[8], const u32 s_raw[8],
-                              u32 prefix, const u32 x_raw[8], const gcomb& G) {
+// One ECDSA verification of item i of batch d; returns the verdict.
+// `slot` is the lane's private table area (kSlotEntries affine entries).
+UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot) {
+    u32 z_raw[8], r_raw[8], s_raw[8], x_raw[8];
+    load_be32_aligned(z_raw, d.msgs + 32ull * i);
+    load_be32_aligned(r_raw, d.sigs + 64ull * i);
+    load_be32_aligned(s_raw, d.sigs + 64ull * i + 32);
+    const u8* pk = d.pubs + 33ull * i;
+    const u32 prefix = pk[0];
+    load_be32_bytes(x_raw, pk + 1);
     bool ok = !u256_is_zero(r_raw) && u256_lt_n(r_raw) && !u256_is_zero(s_raw) && u256_lt_n(s_raw);
     sc s, r, z;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) { s.v[i] = s_raw[i]; r.v[i] = r_raw[i]; z.v[i] = z_raw[i]; }
+    for (int k = 0; k < 8; ++k) { s.v[k] = s_raw[k]; r.v[k] = r_raw[k]; z.v[k] = z_raw[k]; }
     ok &= !sc_is_high(s);
     ok &= (prefix == 2u) | (prefix == 3u);
     aff P;
@@ -92,19 +103,20 @@ UFS_DEV bool ecdsa_verify_one(const u32 z_raw[8], const u32 r_raw[8], const u32 
     sc k1, k2;
     bool n1, n2;
     glv_split(k1, k2, n1, n2, u2);
-    ptable T;
-    build_ptable(T, P);
-    jac RP, RG, R;
-    ecmult_var(RP, T, k1, n1, k2, n2);
-    ecmult_gen(RG, G, u1);
-    jac_add(R, RG, RP);
+    build_ptable(slot, P);
+    jac R;
+    ecmult_dual(R, slot, k1, n1, k2, n2, G, u1);
     const bool v = !R.inf && ecdsa_xcheck(R, r.v);
     return ok && v;
 }
 
-// One BIP-340 verification; all inputs are big-endian words.
-UFS_DEV bool schnorr_verify_one(const u32 m_w[8], const u32 px_w[8], const u32 r_w[8], const u32 s_w[8],
-                                const u32 mid[8], const gcomb& G) {
+// One BIP-340 verification of item i of batch d.
+UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot) {
+    u32 m_w[8], px_w[8], r_w[8], s_w[8];
+    load_be_words(m_w, d.msgs + 32ull * i);
+    load_be_words(px_w, d.pubs + 32ull * i);
+    load_be_words(r_w, d.sigs + 64ull * i);
+    load_be_words(s_w, d.sigs + 64ull * i + 32);
     u32 px[8], rr[8], ss[8];
     u256_from_be_words(px, px_w);
     u256_from_be_words(rr, r_w);
@@ -114,7 +126,7 @@ UFS_DEV bool schnorr_verify_one(const u32 m_w[8], const u32 px_w[8], const u32 r
     ok &= lift_x(P, px, 0u);
     sc s;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) s.v[i] = ss[i];
+    for (int k = 0; k < 8; ++k) s.v[k] = ss[k];
     if (!ok) {
         set_generator(P);
         sc_set_u32(s, 1);
@@ -128,107 +140,77 @@ UFS_DEV bool schnorr_verify_one(const u32 m_w[8], const u32 px_w[8], const u32 r
     sc k1, k2;
     bool n1, n2;
     glv_split(k1, k2, n1, n2, b);
-    ptable T;
-    build_ptable(T, P);
-    jac RP, RG, R;
-    ecmult_var(RP, T, k1, n1, k2, n2);
-    ecmult_gen(RG, G, s);
-    jac_add(R, RG, RP);
+    build_ptable(slot, P);
+    jac R;
+    ecmult_dual(R, slot, k1, n1, k2, n2, G, s);
     if (R.inf) return false;
-    aff A;
-    jac_to_aff(A, R);
-    bool v = (A.y.v[0] & 1u) == 0;
+    // x(R) == r without inversion (r < p): r * Z^2 == X
+    fe z2, t, rf;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v &= A.x.v[i] == rr[i];
+    for (int k = 0; k < 8; ++k) rf.v[k] = rr[k];
+    fe_sqr(z2, R.z);
+    fe_mul(t, rf, z2);
+    bool v = fe_equal(t, R.x);
+    // y(R) even: y = Y / Z^3
+    fe zi, zi3, y;
+    fe_inv(zi, R.z);
+    fe_sqr(zi3, zi);
+    fe_mul(zi3, zi3, zi);
+    fe_mul(y, R.y, zi3);
+    fe_normalize(y);
+    v &= (y.v[0] & 1u) == 0;
     return ok && v;
 }
 
 // ---------------------------------------------------------------------------
-// Kernels
+// Persistent verification kernel.  A wave verifies 64-item tiles; each lane
+// owns one table slot for the kernel's lifetime.  Tiles come from one batch
+// (single mode: wave w takes tiles w, w + W, w + 2W, ...) or from a tile list
+// over uneven batches (list mode: waves pull tiles from a device counter
+// until it passes the tile count, so the grid drains before the kernel
+// returns).  A tile never straddles batches, so a wave runs one signature
+// kind per tile.  KIND: 0 = ECDSA, 1 = BIP-340, 2 = per batch.
 // ---------------------------------------------------------------------------
-template <int MODE>   // 0: write verdict byte, 1: collect (valid -> zero the caller's byte)
-__global__ void __launch_bounds__(kVerifyBlock)
-k_ecdsa_verify(const u8* __restrict__ msgs, const u8* __restrict__ pubs33,
-               const u8* __restrict__ sigs, u32 count, gcomb G, u8* __restrict__ out) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    u32 z[8], r[8], s[8], x[8];
-    load_be32_aligned(z, msgs + 32ull * i);
-    load_be32_aligned(r, sigs + 64ull * i);
-    load_be32_aligned(s, sigs + 64ull * i + 32);
-    const u8* pk = pubs33 + 33ull * i;
-    const u32 prefix = pk[0];
-    load_be32_bytes(x, pk + 1);
-    const bool v = ecdsa_verify_one(z, r, s, prefix, x, G);
-    if (MODE == 0) {
-        out[i] = v ? 1 : 0;
-    } else if (v) {
-        out[i] = 0;
-    }
-}
-
-template <int MODE>
-__global__ void __launch_bounds__(kVerifyBlock)
-k_schnorr_verify(const u8* __restrict__ msgs, const u8* __restrict__ pubx,
-                 const u8* __restrict__ sigs, u32 count, gcomb G,
-                 const u32* __restrict__ mid_g, u8* __restrict__ out) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    u32 m[8], px[8], r[8], s[8], mid[8];
-    load_be_words(m, msgs + 32ull * i);
-    load_be_words(px, pubx + 32ull * i);
-    load_be_words(r, sigs + 64ull * i);
-    load_be_words(s, sigs + 64ull * i + 32);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) mid[j] = mid_g[j];
-    const bool v = schnorr_verify_one(m, px, r, s, mid, G);
-    if (MODE == 0) {
-        out[i] = v ? 1 : 0;
-    } else if (v) {
-        out[i] = 0;
-    }
-}
-
-// Persistent multi-batch verification: each wave pulls 64-item tiles of
-// uneven batches (ECDSA or BIP-340) from a device work counter until the tile
-// list is exhausted.  A tile never straddles batches, so every wave runs one
-// signature kind.  The loop ends for every wave once the counter passes
-// ntiles, so the grid drains before the kernel returns.
-__global__ void __launch_bounds__(kVerifyBlock)
-k_verify_multi(const BatchDesc* __restrict__ descs, const TileDesc* __restrict__ tiles,
-               const u32* __restrict__ ntiles_p, u32* __restrict__ counter, gcomb G, const u32* __restrict__ mid_g) {
+template <int KIND>
+__global__ void __launch_bounds__(kVerifyBlock, UFS_VERIFY_MIN_BLOCKS)
+k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
+    const u32 gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const u32 lane = threadIdx.x & 63u;
-    const u32 ntiles = *ntiles_p;
+    aff* slot = reinterpret_cast<aff*>(a.slots) + (size_t)gtid * kSlotEntries;
     u32 mid[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) mid[j] = mid_g[j];
+    const bool list = a.tiles != nullptr;
+    const u32 ntiles = list ? *a.ntiles_dev : a.single_tiles;
+    const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
+    u32 t = gtid >> 6;
     for (;;) {
-        u32 t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1u);
-        t = __shfl(t, 0, 64);
-        if (t >= ntiles) break;
-        const TileDesc tl = tiles[t];
-        const BatchDesc d = descs[tl.batch];
-        const u32 i = tl.first + lane;
-        if (i >= d.count) continue;
-        bool v;
-        if (d.kind == 0) {
-            u32 z[8], r[8], s[8], x[8];
-            load_be32_aligned(z, d.msgs + 32ull * i);
-            load_be32_aligned(r, d.sigs + 64ull * i);
-            load_be32_aligned(s, d.sigs + 64ull * i + 32);
-            const u8* pk = d.pubs + 33ull * i;
-            load_be32_bytes(x, pk + 1);
-            v = ecdsa_verify_one(z, r, s, pk[0], x, G);
-        } else {
-            u32 m[8], px[8], r[8], s[8];
-            load_be_words(m, d.msgs + 32ull * i);
-            load_be_words(px, d.pubs + 32ull * i);
-            load_be_words(r, d.sigs + 64ull * i);
-            load_be_words(s, d.sigs + 64ull * i + 32);
-            v = schnorr_verify_one(m, px, r, s, mid, G);
+        if (list) {
+            u32 x = 0;
+            if (lane == 0) x = atomicAdd(a.counter, 1u);
+            t = __shfl(x, 0, 64);
         }
-        d.out[i] = v ? 1 : 0;
+        if (t >= ntiles) break;
+        BatchDesc d;
+        u32 first;
+        if (list) {
+            const TileDesc tl = a.tiles[t];
+            d = a.descs[tl.batch];
+            first = tl.first;
+        } else {
+            d = a.single;
+            first = t * kTileItems;
+        }
+        t += nwaves;
+        const u32 i = first + lane;
+        if (i >= d.count) continue;
+        const u32 kind = KIND == 2 ? d.kind : (u32)KIND;
+        const bool v = kind == 0 ? ecdsa_verify_item(d, i, G, slot) : schnorr_verify_item(d, i, mid, G, slot);
+        if (a.collect) {
+            if (v) d.out[i] = 0;
+        } else {
+            d.out[i] = v ? 1 : 0;
+        }
     }
 }
 
@@ -284,7 +266,7 @@ __global__ void k_bip340_midstate(u32* out) {
 // Arithmetic self-test kernel: op codes documented in kernels.h.
 __global__ void __launch_bounds__(64)
 k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __restrict__ out,
-           u32 count, gcomb G) {
+           u32 count, gcomb G, aff* __restrict__ slots) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     fe x, y, r;
@@ -304,6 +286,7 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
     case kOpFeAdd: fe_add(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
     case kOpFeSub: fe_sub(r, x, y); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
     case kOpFeInv: fe_inv(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
+    case kOpFeInvFermat: fe_inv_fermat(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; break;
     case kOpFeSqrt: { bool ok = fe_sqrt(r, x); fe_normalize(r); for (int l = 0; l < 8; ++l) o[l] = r.v[l]; o[8] = ok; } break;
     case kOpFeMulChain:   // throughput probe: 4096 dependent multiplications
         r = x;
@@ -319,6 +302,7 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         break;
     case kOpScMul: sc_mul(sr, sx, sy); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
     case kOpScInv: sc_inv(sr, sx); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
+    case kOpScInvFermat: sc_inv_fermat(sr, sx); for (int l = 0; l < 8; ++l) o[l] = sr.v[l]; break;
     case kOpGlv: {
         sc k1, k2; bool n1, n2;
         glv_split(k1, k2, n1, n2, sx);
@@ -326,8 +310,14 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         o[16] = n1; o[17] = n2;
     } break;
     case kOpGenMul: {   // a*G -> affine (x, y), o[16] = infinity flag
+        aff* slot = slots + (size_t)i * kSlotEntries;
+        aff g;
+        set_generator(g);
+        build_ptable(slot, g);
+        sc zero;
+        sc_set_u32(zero, 0);
         jac R;
-        ecmult_gen(R, G, sx);
+        ecmult_dual(R, slot, zero, false, zero, false, G, sx);
         o[16] = R.inf;
         if (!R.inf) { aff A; jac_to_aff(A, R); for (int l = 0; l < 8; ++l) { o[l] = A.x.v[l]; o[8 + l] = A.y.v[l]; } }
     } break;
@@ -338,10 +328,12 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         bool ok = lift_x(P, xr, 0);
         sc k1, k2; bool n1, n2;
         glv_split(k1, k2, n1, n2, sy);
-        ptable T;
-        build_ptable(T, P);
+        aff* slot = slots + (size_t)i * kSlotEntries;
+        build_ptable(slot, P);
+        sc zero;
+        sc_set_u32(zero, 0);
         jac R;
-        ecmult_var(R, T, k1, n1, k2, n2);
+        ecmult_dual(R, slot, k1, n1, k2, n2, G, zero);
         o[16] = R.inf; o[17] = ok;
         if (!R.inf) { aff A; jac_to_aff(A, R); for (int l = 0; l < 8; ++l) { o[l] = A.x.v[l]; o[8 + l] = A.y.v[l]; } }
     } break;
@@ -358,46 +350,28 @@ namespace ufs {
 
 static inline unsigned grid_for(u32 count, unsigned block) { return (count + block - 1) / block; }
 
-hipError_t launch_ecdsa_verify(const u8* msgs, const u8* pubs33, const u8* sigs, u32 count,
-                               const GcombDesc& g, u8* out, bool collect, hipStream_t st) {
-    if (count == 0) return hipSuccess;
+hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, const uint32_t* mid, unsigned grid,
+                         hipStream_t st) {
+    if (grid == 0) return hipSuccess;
     gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
-    if (collect)
-        hipLaunchKernelGGL(k_ecdsa_verify<1>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
-                           msgs, pubs33, sigs, count, G, out);
-    else
-        hipLaunchKernelGGL(k_ecdsa_verify<0>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
-                           msgs, pubs33, sigs, count, G, out);
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(k_verify<0>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
+    case 1: hipLaunchKernelGGL(k_verify<1>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
+    default: hipLaunchKernelGGL(k_verify<2>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_schnorr_verify(const u8* msgs, const u8* pubx, const u8* sigs, u32 count,
-                                 const GcombDesc& g, const u32* mid, u8* out, bool collect, hipStream_t st) {
-    if (count == 0) return hipSuccess;
-    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
-    if (collect)
-        hipLaunchKernelGGL(k_schnorr_verify<1>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
-                           msgs, pubx, sigs, count, G, mid, out);
-    else
-        hipLaunchKernelGGL(k_schnorr_verify<0>, dim3(grid_for(count, kVerifyBlock)), dim3(kVerifyBlock), 0, st,
-                           msgs, pubx, sigs, count, G, mid, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_verify_multi(const BatchDesc* descs, const TileDesc* tiles, const uint32_t* ntiles,
-                               uint32_t* counter, const GcombDesc& g, const uint32_t* mid, unsigned grid_blocks,
-                               hipStream_t st) {
-    gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
-    hipLaunchKernelGGL(k_verify_multi, dim3(grid_blocks), dim3(kVerifyBlock), 0, st, descs, tiles, ntiles, counter,
-                       G, mid);
-    return hipGetLastError();
-}
-
-int verify_multi_blocks_per_cu() {
+int verify_blocks_per_cu(int kind) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify_multi, kVerifyBlock, 0) != hipSuccess)
-        return 1;
-    return blocks > 0 ? blocks : 1;
+    hipError_t e;
+    switch (kind) {
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<0>, kVerifyBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<1>, kVerifyBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<2>, kVerifyBlock, 0); break;
+    }
+    if (e != hipSuccess || blocks <= 0) return 1;
+    return blocks;
 }
 
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st) {
@@ -413,10 +387,11 @@ hipError_t launch_bip340_midstate(u32* out, hipStream_t st) {
 }
 
 hipError_t launch_selftest(int op, const u32* a, const u32* b, u32* out, u32 count,
-                           const GcombDesc& g, hipStream_t st) {
+                           const GcombDesc& g, void* slots, hipStream_t st) {
     if (count == 0) return hipSuccess;
     gcomb G{reinterpret_cast<const aff*>(g.table), g.w, g.nwin};
-    hipLaunchKernelGGL(k_selftest, dim3(grid_for(count, 64)), dim3(64), 0, st, op, a, b, out, count, G);
+    hipLaunchKernelGGL(k_selftest, dim3(grid_for(count, 64)), dim3(64), 0, st, op, a, b, out, count, G,
+                       reinterpret_cast<aff*>(slots));
     return hipGetLastError();
 }
 

@@ -111,15 +111,49 @@ UFS_DEV void odd_digit(const u32 k[8], int i, int w, u32& idx, bool& neg) {
 }
 
 // ---------------------------------------------------------------------------
-// Variable-base table: odd multiples {1,3,...,15}*P with a common Z.
+// Variable-base table: odd multiples {1,3,...,15}*P with a common Z, kept in
+// a lane-private slot of device memory rather than in VGPRs (eight affine
+// points are 128 VGPRs, which alone would cap the kernels at one wave per
+// SIMD).  Each slot is 768 bytes, contiguous per lane, so one entry read is
+// four 16-byte loads of a single 64-byte line segment:
+//   slot[0..7]  (2j+1)*P' on the isomorphic curve (affine, 64 B each)
+//   slot[8]     x = zs, the Z scale of the isomorphic curve
+//   slot[9..11] Z ratios used while the table is built
+// The entries for a window are loaded before its doublings, so the load
+// latency hides behind four doublings.
 // ---------------------------------------------------------------------------
-struct ptable {
-    fe x[kPEntries];
-    fe y[kPEntries];
-    fe zs;   // true Z of the accumulator = acc.z * zs
-};
+constexpr int kSlotEntries = 12;   // 768 bytes per lane
 
-UFS_DEV void build_ptable(ptable& T, const aff& P) {
+UFS_DEV void store_fe(fe* p, const fe& a) {
+    uint4* q = reinterpret_cast<uint4*>(p);
+    q[0] = make_uint4(a.v[0], a.v[1], a.v[2], a.v[3]);
+    q[1] = make_uint4(a.v[4], a.v[5], a.v[6], a.v[7]);
+}
+
+UFS_DEV void load_fe(fe& a, const fe* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    const uint4 x = q[0], y = q[1];
+    a.v[0] = x.x; a.v[1] = x.y; a.v[2] = x.z; a.v[3] = x.w;
+    a.v[4] = y.x; a.v[5] = y.y; a.v[6] = y.z; a.v[7] = y.w;
+}
+
+UFS_DEV void store_aff(aff* p, const aff& a) {
+    store_fe(&p->x, a.x);
+    store_fe(&p->y, a.y);
+}
+
+UFS_DEV void load_aff(aff& r, const aff* p) {
+    load_fe(r.x, &p->x);
+    load_fe(r.y, &p->y);
+}
+
+// Z ratio zr[j] (j = 1..7) lives in the slot's spare entries.
+UFS_DEV fe* slot_zr(aff* slot, int j) {
+    fe* z = reinterpret_cast<fe*>(slot + 9);
+    return z + (j - 1);
+}
+
+UFS_DEV void build_ptable(aff* slot, const aff& P) {
     jac Pj, D;
     jac_from_aff(Pj, P);
     jac_dbl(D, Pj);
@@ -131,10 +165,8 @@ UFS_DEV void build_ptable(ptable& T, const aff& P) {
     fe_mul(cx, P.x, z2);
     fe_mul(cy, P.y, z3);
     fe_set_u32(cz, 1);
-    T.x[0] = cx;
-    T.y[0] = cy;
-    fe zr[kPEntries];
-#pragma unroll
+    store_aff(slot, aff{cx, cy});
+    fe zlast;
     for (int j = 1; j < kPEntries; ++j) {
         // (cx,cy,cz) += (D.x, D.y) -- madd-2007-bl; no exceptional case can
         // arise: (2j-1)P = +-2P is impossible for a point of order n.
@@ -161,39 +193,36 @@ UFS_DEV void build_ptable(ptable& T, const aff& P) {
         fe_mul(jj, cy, jj);
         fe_add(jj, jj, jj);
         fe_sub(ny, t, jj);
-        fe_add(zr[j], h, h);       // Z_j = Z_{j-1} * 2H
-        fe_mul(cz, cz, zr[j]);
+        fe zr;
+        fe_add(zr, h, h);          // Z_j = Z_{j-1} * 2H
+        fe_mul(cz, cz, zr);
         cx = nx;
         cy = ny;
-        T.x[j] = cx;
-        T.y[j] = cy;
+        store_aff(slot + j, aff{cx, cy});
+        if (j < kPEntries - 1) store_fe(slot_zr(slot, j), zr);
+        zlast = zr;
     }
-    // Rescale entries 0..6 to the common Z = cz.
-    fe s = zr[kPEntries - 1];
-#pragma unroll
+    // Rescale entries 0..6 to the common Z = cz: entry j gets the product of
+    // the ratios after it, s = zr[j+1] * ... * zr[7].
+    fe s = zlast;
     for (int j = kPEntries - 2; j >= 0; --j) {
         fe s2, s3;
+        aff e;
+        load_aff(e, slot + j);
         fe_sqr(s2, s);
         fe_mul(s3, s2, s);
-        fe_mul(T.x[j], T.x[j], s2);
-        fe_mul(T.y[j], T.y[j], s3);
-        if (j > 0) fe_mul(s, s, zr[j]);
-    }
-    fe_mul(T.zs, cz, D.z);
-}
-
-UFS_DEV void ptable_select(aff& r, const ptable& T, u32 idx) {
-    r.x = T.x[0];
-    r.y = T.y[0];
-#pragma unroll
-    for (int j = 1; j < kPEntries; ++j) {
-        const bool c = idx == (u32)j;
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-            r.x.v[l] = c ? T.x[j].v[l] : r.x.v[l];
-            r.y.v[l] = c ? T.y[j].v[l] : r.y.v[l];
+        fe_mul(e.x, e.x, s2);
+        fe_mul(e.y, e.y, s3);
+        store_aff(slot + j, e);
+        if (j > 0) {
+            fe zr;
+            load_fe(zr, slot_zr(slot, j));
+            fe_mul(s, s, zr);
         }
     }
+    fe zs;
+    fe_mul(zs, cz, D.z);
+    store_fe(&slot[8].x, zs);
 }
 
 UFS_DEV void fe_cond_neg(fe& y, bool c) {
@@ -201,55 +230,6 @@ UFS_DEV void fe_cond_neg(fe& y, bool c) {
     fe_neg(n, y);
 #pragma unroll
     for (int l = 0; l < 8; ++l) y.v[l] = c ? n.v[l] : y.v[l];
-}
-
-// acc (on the isomorphic curve) = k1*P' + k2*phi(P') where P' = +-P by sign.
-// Returns the true Jacobian point (Z multiplied by T.zs).
-UFS_DEV void ecmult_var(jac& out, const ptable& T, sc k1, bool neg1, sc k2, bool neg2) {
-    const fe BETA = {{UFS_BETA_LIMBS}};
-    const bool fix1 = make_odd(k1);
-    const bool fix2 = make_odd(k2);
-    jac acc;
-    aff e;
-    // top digits are 1 for both streams (k < 2^128)
-    e.x = T.x[0];
-    e.y = T.y[0];
-    fe_cond_neg(e.y, neg1);
-    jac_from_aff(acc, e);
-    fe_mul(e.x, T.x[0], BETA);
-    e.y = T.y[0];
-    fe_cond_neg(e.y, neg2);
-    jac_madd(acc, acc, e);
-    for (int i = kPWindows - 1; i >= 0; --i) {
-        for (int d = 0; d < kPW; ++d) jac_dbl(acc, acc);
-        u32 idx;
-        bool ng;
-        odd_digit(k1.v, i, kPW, idx, ng);
-        ptable_select(e, T, idx);
-        fe_cond_neg(e.y, ng != neg1);
-        jac_madd(acc, acc, e);
-        odd_digit(k2.v, i, kPW, idx, ng);
-        ptable_select(e, T, idx);
-        fe_mul(e.x, e.x, BETA);
-        fe_cond_neg(e.y, ng != neg2);
-        jac_madd(acc, acc, e);
-    }
-    // corrections for even scalars: subtract one P' (resp. phi(P'))
-    {
-        jac t;
-        e.x = T.x[0];
-        e.y = T.y[0];
-        fe_cond_neg(e.y, !neg1);
-        jac_madd(t, acc, e);
-        jac_select(acc, fix1, t);
-        fe_mul(e.x, T.x[0], BETA);
-        e.y = T.y[0];
-        fe_cond_neg(e.y, !neg2);
-        jac_madd(t, acc, e);
-        jac_select(acc, fix2, t);
-    }
-    out = acc;
-    fe_mul(out.z, acc.z, T.zs);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,52 +243,124 @@ struct gcomb {
     int nwin;
 };
 
-UFS_DEV void load_aff(aff& r, const aff* p) {
-    const uint4* q = reinterpret_cast<const uint4*>(p);
-    uint4 a = q[0], b = q[1], c = q[2], d = q[3];
-    r.x.v[0] = a.x; r.x.v[1] = a.y; r.x.v[2] = a.z; r.x.v[3] = a.w;
-    r.x.v[4] = b.x; r.x.v[5] = b.y; r.x.v[6] = b.z; r.x.v[7] = b.w;
-    r.y.v[0] = c.x; r.y.v[1] = c.y; r.y.v[2] = c.z; r.y.v[3] = c.w;
-    r.y.v[4] = d.x; r.y.v[5] = d.y; r.y.v[6] = d.z; r.y.v[7] = d.w;
-}
-
-// r = a*G for any a in [0, n).
-UFS_DEV void ecmult_gen(jac& r, const gcomb& G, const sc& a) {
+// ---------------------------------------------------------------------------
+// Double-scalar multiplication u*G + (k1 + lambda k2)*P' as one flat loop of
+// steps, so a single inline copy of the doubling and of the mixed addition
+// serves the whole computation (small code, no calls on the hot path, free
+// register allocation).  Every lane runs the same step sequence:
+//   steps 0..191   window i = 31..0 of the GLV halves: 4 doublings, then the
+//                  digit of k1 (+-P') and of k2 (+-phi(P')) from the slot;
+//   steps 192,193  corrections for even k1, k2 (subtract P' / phi(P'));
+//   step 194       rescale from the isomorphic curve: z *= zs;
+//   steps 195..    the G comb: top entry, then one entry per window.
+// A mixed addition that meets a == b (possible only for adversarial inputs)
+// marks the lane; the next step is then a doubling for the marked lanes
+// before the schedule continues.
+// ---------------------------------------------------------------------------
+UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, bool neg2,
+                         const gcomb& G, const sc& u) {
     const u32 N[8] = {UFS_N_LIMBS};
-    // odd representative: a odd -> a, else n - a (odd), negating G.
-    sc k;
-    const bool even = (a.v[0] & 1u) == 0;
+#pragma unroll
+    for (int i = 4; i < 8; ++i) {
+        k1.v[i] = 0;
+        k2.v[i] = 0;
+    }
+    const bool fix1 = make_odd(k1);
+    const bool fix2 = make_odd(k2);
+    const u32 f1 = neg1 ? kAddNeg : 0u, f2 = (neg2 ? kAddNeg : 0u) | kAddBeta;
+    // odd representative of u: u odd -> u, else n - u (odd), negating G
+    sc ku;
+    const bool ueven = (u.v[0] & 1u) == 0;
     {
         u32 bw = 0, t[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) t[i] = subc(N[i], a.v[i], bw, bw);
+        for (int i = 0; i < 8; ++i) t[i] = subc(N[i], u.v[i], bw, bw);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) k.v[i] = even ? t[i] : a.v[i];
+        for (int i = 0; i < 8; ++i) ku.v[i] = ueven ? t[i] : u.v[i];
     }
+    const u32 fg = ueven ? kAddNeg : 0u;
     const int half = 1 << (G.w - 1);
-    aff e;
-    load_aff(e, G.tab + (size_t)G.nwin * half);
-    fe_cond_neg(e.y, even);
+    constexpr int kVarSteps = kPWindows * (kPW + 2);
+    const int nsteps = kVarSteps + 3 + 1 + G.nwin;
+
+    // accumulator starts at the top digits, P' + phi(P') (both 1: k < 2^128):
+    // P' itself here, phi(P') as the first scheduled addition below.
     jac acc;
-    jac_from_aff(acc, e);
-    u32 idx;
-    bool ng;
-    odd_digit(k.v, 0, G.w, idx, ng);
-    aff nxt;
-    load_aff(nxt, G.tab + idx);
-    bool nxt_neg = ng;
-    for (int i = 0; i < G.nwin; ++i) {
-        e = nxt;
-        const bool eneg = nxt_neg != even;
-        if (i + 1 < G.nwin) {
-            odd_digit(k.v, i + 1, G.w, idx, ng);
-            load_aff(nxt, G.tab + (size_t)(i + 1) * half + idx);
-            nxt_neg = ng;
-        }
-        fe_cond_neg(e.y, eneg);
-        jac_madd(acc, acc, e);
+    {
+        aff e;
+        load_aff(e, slot);
+        fe_cond_neg(e.y, neg1);
+        jac_from_aff(acc, e);
     }
-    r = acc;
+    bool pending = false;
+    int step = -1;   // step -1 adds phi(P') for the top digit of k2
+    while (step < nsteps) {
+        const bool fixup = __any(pending);
+        bool is_dbl = fixup;
+        int r = 0;
+        if (!fixup && step >= 0 && step < kVarSteps) {
+            r = step % (kPW + 2);
+            is_dbl = r < kPW;
+        }
+        if (is_dbl) {
+            jac t;
+            jac_dbl_body<true>(t, acc);
+            if (fixup) {
+                if (pending) acc = t;
+                pending = false;
+                continue;
+            }
+            acc = t;
+        } else if (step == kVarSteps + 2) {
+            fe zs;
+            load_fe(zs, &slot[8].x);
+            fe_mul(acc.z, acc.z, zs);
+        } else {
+            const aff* bp;
+            u32 fl;
+            bool keep = true;
+            if (step < 0) {
+                bp = slot;
+                fl = f2;
+            } else if (step < kVarSteps) {
+                const int w = kPWindows - 1 - step / (kPW + 2);
+                u32 idx;
+                bool ng;
+                if (r == kPW) {
+                    odd_digit(k1.v, w, kPW, idx, ng);
+                    fl = ng ? (f1 ^ kAddNeg) : f1;
+                } else {
+                    odd_digit(k2.v, w, kPW, idx, ng);
+                    fl = ng ? (f2 ^ kAddNeg) : f2;
+                }
+                bp = slot + idx;
+            } else if (step < kVarSteps + 2) {
+                const bool first = step == kVarSteps;
+                bp = slot;
+                fl = (first ? f1 : f2) ^ kAddNeg;
+                keep = first ? fix1 : fix2;
+            } else if (step == kVarSteps + 3) {
+                bp = G.tab + (size_t)G.nwin * half;
+                fl = fg;
+            } else {
+                const int i = step - (kVarSteps + 4);
+                u32 idx;
+                bool ng;
+                odd_digit(ku.v, i, G.w, idx, ng);
+                bp = G.tab + (size_t)i * half + idx;
+                fl = ng ? (fg ^ kAddNeg) : fg;
+            }
+            jac t;
+            bool nd;
+            jac_madd_mem_body(t, acc, bp, fl, nd);
+            if (keep) {
+                acc = t;
+                pending = nd;
+            }
+        }
+        ++step;
+    }
+    out = acc;
 }
 
 }  // namespace ufs

@@ -270,9 +270,22 @@ UFS_DEV void fe_reduce512(fe& r, const u32 t[16]) {
     fe_fold(r, c);
 }
 
-// Multiplication and squaring are kept out of line (arguments and results
-// travel in VGPRs): one copy of each body keeps the point-arithmetic loops
-// small enough for the instruction cache.
+// Inline bodies, used inside the out-of-line point operations.
+UFS_DEV void fe_mul_i(fe& r, const fe& a, const fe& b) {
+    u32 t[16];
+    mul_512(t, a.v, b.v);
+    fe_reduce512(r, t);
+}
+
+UFS_DEV void fe_sqr_i(fe& r, const fe& a) {
+    u32 t[16];
+    sqr_512(t, a.v);
+    fe_reduce512(r, t);
+}
+
+// Out-of-line multiplication and squaring for everything else (arguments
+// and results travel in VGPRs): one copy of each body keeps the code small
+// enough for the instruction cache.
 __device__ __noinline__ fe fe_mul_v(fe a, fe b) {
     u32 t[16];
     fe r;
@@ -409,8 +422,9 @@ UFS_DEV void fe_pow_x223(fe& x223, fe& x22, fe& x2, const fe& a) {
     fe_sqr_n(t, x220, 3); fe_mul(x223, t, x3);
 }
 
-// a^(p-2): 255 squarings + 15 multiplications.
-UFS_DEV void fe_inv(fe& r, const fe& a) {
+// a^(p-2): 255 squarings + 15 multiplications.  The kernels invert with
+// divsteps (fe_inv); this form is kept as an independent cross-check.
+UFS_DEV void fe_inv_fermat(fe& r, const fe& a) {
     fe x223, x22, x2, t;
     fe_pow_x223(x223, x22, x2, a);
     fe_sqr_n(t, x223, 23); fe_mul(t, t, x22);
@@ -650,8 +664,9 @@ UFS_DEV void sc_sqr_n(sc& r, const sc& a, int n) {
     r = t;
 }
 
-// r = a^(n-2) mod n (Fermat inverse; a != 0).
-UFS_DEV void sc_inv(sc& r, const sc& a) {
+// r = a^(n-2) mod n (Fermat inverse).  The kernels invert with divsteps
+// (sc_inv); this form is kept as an independent cross-check.
+UFS_DEV void sc_inv_fermat(sc& r, const sc& a) {
     sc pw[8];   // a^1, a^3, ..., a^15
     sc a2, t;
     sc_sqr(a2, a);

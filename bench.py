@@ -38,6 +38,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kind: str, n: int):
+    """HBM traffic per launch of the verify kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there); None when no summary covers the workload."""
+    if n != 1 << 20:
+        return None, None
+    p = ROOT / "profiles" / f"k_verify_{kind}_2p20.md"
+    if not p.is_file():
+        return None, None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1]), str(p.relative_to(ROOT))
+    return None, None
+
+
 def measure_valu_peak() -> dict:
     """Run the v_mad_u64_u32 throughput probe; returns its JSON records."""
     exe = ROOT / "ultrafastsecp256k1_amd" / "valu_peak"
@@ -236,9 +251,10 @@ def main() -> int:
     mad = peak["v_mad_u64_u32"]["ops_per_s"]
     w = W_SCHNORR if kind == "schnorr" else W_ECDSA
     achieved = head["kernel_sigs_per_s"] * w
+    traffic, traffic_src = profiled_traffic(kind, n)
     roofline = {"bound": "valu", "unit": "TMAC32/s", "achieved": achieved / 1e12, "peak": mad / 1e12,
-                "frac": achieved / mad, "traffic": None,
-                "kernel": f"k_{kind}_verify<0>", "kernel_ms": round(head["kernel_ms"], 4),
+                "frac": achieved / mad, "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": f"k_verify<{0 if kind == 'ecdsa' else 1}> ({kind})", "kernel_ms": round(head["kernel_ms"], 4),
                 "work_per_item_mac32": w,
                 "hbm_gbps": n * (BYTES_SCHNORR if kind == "schnorr" else BYTES_ECDSA) / (head["kernel_ms"] * 1e-3) / 1e9,
                 "peak_source": "valu_peak probe (v_mad_u64_u32, measured on this GPU)"}

@@ -20,6 +20,8 @@
 
 namespace ufs {
 
+static_assert(kSlotEntries * sizeof(aff) == kSlotBytes, "lane slot size");
+
 UFS_DEV void load_be32_aligned(u32 r[8], const u8* p) {
     const uint4* q = reinterpret_cast<const uint4*>(p);
     const uint4 a = q[0], b = q[1];

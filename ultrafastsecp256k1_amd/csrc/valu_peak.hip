@@ -6,6 +6,10 @@
 //
 //   valu_peak [iters]  ->  one JSON line per instruction:
 //     {"inst": ..., "ops_per_s": ..., "per_cu_per_clk": ...}
+//   valu_peak copy <MiB>  ->  one plain device-to-device copy of that size
+//     (16-byte loads and stores), the calibration kernel for the FETCH_SIZE /
+//     WRITE_SIZE counters: its algorithmic traffic is exactly MiB read + MiB
+//     written.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -125,7 +129,36 @@ static void run(const char* name, int iters, int blocks, double clk_ghz, int cus
     CHK(hipEventDestroy(e1));
 }
 
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+static int run_copy(int mib) {
+    const size_t bytes = (size_t)mib << 20, n = bytes / 16;
+    uint4 *src, *dst;
+    CHK(hipMalloc(&src, bytes));
+    CHK(hipMalloc(&dst, bytes));
+    CHK(hipMemset(src, 1, bytes));
+    hipEvent_t e0, e1;
+    CHK(hipEventCreate(&e0));
+    CHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, 0, src, dst, n);   // warm-up
+    CHK(hipDeviceSynchronize());
+    CHK(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, 0, src, dst, n);
+    CHK(hipEventRecord(e1, 0));
+    CHK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHK(hipEventElapsedTime(&ms, e0, e1));
+    std::printf("{\"copy_bytes\": %zu, \"ms\": %.4f, \"gbps\": %.1f}\n", bytes, ms, 2.0 * bytes / (ms * 1e-3) / 1e9);
+    CHK(hipFree(src));
+    CHK(hipFree(dst));
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 2 && argv[1][0] == 'c') return run_copy(std::atoi(argv[2]));
     const int iters = argc > 1 ? std::atoi(argv[1]) : 4096;
     hipDeviceProp_t p;
     CHK(hipGetDeviceProperties(&p, 0));

@@ -99,6 +99,67 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_synchronize(ufsecp_gpu_ctx* ctx);
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_selftest(ufsecp_gpu_ctx* ctx, int op, const uint32_t* a, const uint32_t* b,
                                                   uint32_t* out, size_t count);
 
+/* Pinned (page-locked) host memory: inputs submitted from such buffers are
+ * copied at full DMA rate.  Not a device allocation. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** h_ptr_out);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_host_free(ufsecp_gpu_ctx* ctx, void* h_ptr);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop: continuous batching of verify requests.
+ *
+ * Requests of either kind and any size (1 .. max_group_items) are submitted
+ * from any thread and complete asynchronously.  Each request is admitted when
+ * its inputs fit in the server's device memory budget (FIFO; waiting requests
+ * are admitted as earlier ones complete), and every admitted request joins the
+ * next launch group: one persistent launch over all of them, with public keys
+ * resolved through a device-resident decompression cache.  Up to
+ * `inflight_groups` groups are on the device at once.  Verdicts (1 byte per
+ * item, as ufsecp_gpu_*_verify_batch) are in out_results once poll/wait
+ * report completion; the input and output buffers must stay valid until then.
+ * The context must outlive the server.  Public data only.
+ * ------------------------------------------------------------------------- */
+typedef struct ufsecp_gpu_server ufsecp_gpu_server;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;   /* device bytes for in-flight inputs + keys (0 = 256 MiB) */
+    uint32_t inflight_groups;    /* launch groups on the device at once, 1..4 (0 = 2) */
+    uint32_t key_cache_log2;     /* decompression cache of 2^k 64-byte entries; 0 = none */
+    uint32_t max_group_items;    /* items coalesced into one launch (0 = 2^21) */
+    uint32_t reserved;
+} ufsecp_gpu_server_config_t;
+
+typedef struct {
+    uint64_t requests_completed;
+    uint64_t items_completed;
+    uint64_t groups_launched;
+    uint64_t key_lookups;
+    uint64_t key_cache_hits;
+    double latency_p50_us;       /* submit -> verdicts in the caller's buffer */
+    double latency_p99_us;
+    double latency_mean_us;
+    double latency_max_us;
+    double busy_seconds;         /* wall time with at least one group in flight */
+} ufsecp_gpu_server_stats_t;
+
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_create(ufsecp_gpu_ctx* ctx, const ufsecp_gpu_server_config_t* cfg,
+                                                       ufsecp_gpu_server** server_out);
+/* Stops the server; call after ufsecp_gpu_ext_server_drain. */
+UFSECP_API void ufsecp_gpu_ext_server_destroy(ufsecp_gpu_server* server);
+/* kind: UFSECP_GPU_KIND_ECDSA (pubkeys N x 33) or UFSECP_GPU_KIND_SCHNORR
+ * (x-only pubkeys N x 32); msgs N x 32, sigs N x 64, out_results N bytes. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_submit(ufsecp_gpu_server* server, uint32_t kind,
+                                                       const uint8_t* msg_hashes32, const uint8_t* pubkeys,
+                                                       const uint8_t* sigs64, size_t count, uint8_t* out_results,
+                                                       uint64_t* request_id);
+/* 1: done (the id is then forgotten), 0: pending, -1: unknown id. */
+UFSECP_API int ufsecp_gpu_ext_server_poll(ufsecp_gpu_server* server, uint64_t request_id);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_wait(ufsecp_gpu_server* server, uint64_t request_id);
+/* Waits for every submitted request and forgets all completed ids. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_drain(ufsecp_gpu_server* server);
+/* Counters and latency percentiles since creation or the last reset. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_stats(ufsecp_gpu_server* server, ufsecp_gpu_server_stats_t* out,
+                                                      int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,3 +95,14 @@ def test_textbook_helpers_consistent():
     # the Python helpers used to pin device primitives agree with closed forms
     assert mul(N - 1) == (G[0], P - G[1])
     assert lift_x(G[0], odd=bool(G[1] & 1)) == G
+
+
+@pytest.mark.parametrize("kind", ["ecdsa", "schnorr"])
+def test_pool_workload_items_verify(kind):
+    """Serving-workload items (keys partly from a pool) are valid signatures
+    except the corrupted ones, and pool keys really repeat."""
+    b = wl.generate_pool(kind, 600, 0x51, 0x52, pool_size=8, pool_permille=200)
+    v = orc.verify_batch(kind, b.msgs, b.pubkeys, b.sigs)
+    assert np.array_equal(v, 1 - b.corrupted)
+    _, counts = np.unique(b.pubkeys, axis=0, return_counts=True)
+    assert counts.max() > 5

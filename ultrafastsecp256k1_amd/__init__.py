@@ -43,6 +43,12 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_stream", "ufsecp_gpu_ext_synchronize", "ufsecp_gpu_ext_selftest",
     "ufsecp_gpu_ext_verify_multi_device", "ufsecp_gpu_ext_verify_each_device",
     "ufsecp_gpu_ext_msm", "ufsecp_gpu_ext_msm_device",
+    "ufsecp_gpu_ext_alloc", "ufsecp_gpu_ext_free", "ufsecp_gpu_ext_copy_to_device", "ufsecp_gpu_ext_copy_to_host",
+    "ufsecp_gpu_ext_event_create", "ufsecp_gpu_ext_event_record", "ufsecp_gpu_ext_event_elapsed",
+    "ufsecp_gpu_ext_event_destroy", "ufsecp_gpu_ext_host_alloc", "ufsecp_gpu_ext_host_free",
+    "ufsecp_gpu_ext_server_create", "ufsecp_gpu_ext_server_destroy", "ufsecp_gpu_ext_server_submit",
+    "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
+    "ufsecp_gpu_ext_server_stats",
 )
 
 
@@ -54,6 +60,24 @@ class VerifyBatchDesc(ctypes.Structure):
 
 KIND_ECDSA = 0
 KIND_SCHNORR = 1
+
+
+class ServerConfig(ctypes.Structure):
+    """ufsecp_gpu_server_config_t (include/ufsecp/ufsecp_gpu_ext.h)."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("inflight_groups", c_uint32),
+                ("key_cache_log2", c_uint32), ("max_group_items", c_uint32), ("reserved", c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    """ufsecp_gpu_server_stats_t (include/ufsecp/ufsecp_gpu_ext.h)."""
+    _fields_ = [("requests_completed", ctypes.c_uint64), ("items_completed", ctypes.c_uint64),
+                ("groups_launched", ctypes.c_uint64), ("key_lookups", ctypes.c_uint64),
+                ("key_cache_hits", ctypes.c_uint64), ("latency_p50_us", ctypes.c_double),
+                ("latency_p99_us", ctypes.c_double), ("latency_mean_us", ctypes.c_double),
+                ("latency_max_us", ctypes.c_double), ("busy_seconds", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class UfsecpError(RuntimeError):
@@ -118,6 +142,16 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_stream": (vp, [vp]),
         "ufsecp_gpu_ext_synchronize": (c_int, [vp]),
         "ufsecp_gpu_ext_selftest": (c_int, [vp, c_int, vp, vp, vp, c_size_t]),
+        "ufsecp_gpu_ext_host_alloc": (c_int, [vp, c_size_t, POINTER(vp)]),
+        "ufsecp_gpu_ext_host_free": (c_int, [vp, vp]),
+        "ufsecp_gpu_ext_server_create": (c_int, [vp, vp, POINTER(vp)]),
+        "ufsecp_gpu_ext_server_destroy": (None, [vp]),
+        "ufsecp_gpu_ext_server_submit": (c_int, [vp, c_uint32, vp, vp, vp, c_size_t, vp,
+                                                 POINTER(ctypes.c_uint64)]),
+        "ufsecp_gpu_ext_server_poll": (c_int, [vp, ctypes.c_uint64]),
+        "ufsecp_gpu_ext_server_wait": (c_int, [vp, ctypes.c_uint64]),
+        "ufsecp_gpu_ext_server_drain": (c_int, [vp]),
+        "ufsecp_gpu_ext_server_stats": (c_int, [vp, vp, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -306,6 +340,77 @@ class UfsecpGpu:
         rc = self._lib.ufsecp_gpu_ext_selftest(self._ctx, op, _buf(a), _buf(b), _buf(out), n)
         self._check("ufsecp_gpu_ext_selftest", rc)
         return out
+
+
+class VerifyServer:
+    """Continuous-batching verification server over a context
+    (ufsecp_gpu_ext_server_*).  Submitted numpy buffers are kept referenced
+    until the request is observed complete."""
+
+    def __init__(self, gpu: UfsecpGpu, hbm_budget_bytes: int = 0, inflight_groups: int = 0,
+                 key_cache_log2: int = 21, max_group_items: int = 0):
+        self.gpu = gpu
+        self._lib = gpu.lib
+        cfg = ServerConfig(hbm_budget_bytes, inflight_groups, key_cache_log2, max_group_items, 0)
+        h = c_void_p()
+        rc = self._lib.ufsecp_gpu_ext_server_create(gpu.ctx, byref(cfg), byref(h))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_server_create", rc)
+        self._h = h
+        self._keep = {}
+
+    def submit(self, kind: int, msgs, pubkeys, sigs, out) -> int:
+        n = _nitems(msgs, 32)
+        rid = ctypes.c_uint64()
+        rc = self._lib.ufsecp_gpu_ext_server_submit(self._h, kind, _buf(msgs), _buf(pubkeys), _buf(sigs), n,
+                                                    _buf(out), byref(rid))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_server_submit", rc)
+        self._keep[rid.value] = (msgs, pubkeys, sigs, out)
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        r = self._lib.ufsecp_gpu_ext_server_poll(self._h, rid)
+        if r == 1:
+            self._keep.pop(rid, None)
+        return r
+
+    def wait(self, rid: int) -> None:
+        rc = self._lib.ufsecp_gpu_ext_server_wait(self._h, rid)
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_server_wait", rc)
+        self._keep.pop(rid, None)
+
+    def drain(self) -> None:
+        rc = self._lib.ufsecp_gpu_ext_server_drain(self._h)
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_server_drain", rc)
+        self._keep.clear()
+
+    def stats(self, reset: bool = False) -> dict:
+        st = ServerStats()
+        rc = self._lib.ufsecp_gpu_ext_server_stats(self._h, byref(st), 1 if reset else 0)
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_server_stats", rc)
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ufsecp_gpu_ext_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *_):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceArray:

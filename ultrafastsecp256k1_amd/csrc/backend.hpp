@@ -96,6 +96,9 @@ public:
 
     const GcombDesc& gcomb() const { return gdesc_; }
     const uint32_t* bip340_midstate() const { return d_mid_; }
+    int device() const { return device_; }
+    int compute_units() const { return cus_; }
+    unsigned verify_grid(int kind) const { return verify_grid_[kind]; }
 
     GpuError fail(GpuError e, const char* what, hipError_t he = hipSuccess);
     void clear_error() {

@@ -74,6 +74,11 @@ struct VerifyLaunch {
     const uint32_t* ntiles_dev;
     uint32_t* counter;
     void* slots;                // kSlotBytes per lane of the grid
+    // Optional public keys already decompressed (64-byte affine points per
+    // item, y = all-ones bits for an invalid key): per batch in list mode,
+    // `single_pts` in single mode; nullptr = decompress in the kernel.
+    const void* const* batch_pts;
+    const void* single_pts;
     uint32_t single_tiles;
     uint32_t collect;           // 1: valid -> zero the caller's byte, else keep
 };
@@ -83,6 +88,25 @@ hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, co
                          hipStream_t st);
 int verify_blocks_per_cu(int kind);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+
+// Public-key decompression through a device-resident cache (serving path).
+// The cache is a direct-mapped table of (x, y with even parity) entries,
+// 2^k entries of 64 bytes; an entry is trusted only if its x matches and
+// y^2 == x^3 + 7, so concurrent writers can never produce a wrong point.
+struct KeyCacheDesc {
+    void* entries;
+    uint32_t mask;   // entries - 1
+};
+// Resolve every item of the tile list: cache hits and invalid keys are written
+// to the batch's points array; misses are appended to `miss` as (batch, item)
+// pairs, counted in *miss_count (zero at launch).
+hipError_t launch_key_resolve(const BatchDesc* descs, const TileDesc* tiles, uint32_t ntiles,
+                              const void* const* batch_pts, KeyCacheDesc cache, uint32_t* miss, uint32_t* miss_count,
+                              hipStream_t st);
+// Decompress the misses (grid-stride over *miss_count), write the points and
+// insert them into the cache.
+hipError_t launch_key_decompress(const BatchDesc* descs, const void* const* batch_pts, KeyCacheDesc cache,
+                                 const uint32_t* miss, const uint32_t* miss_count, unsigned grid, hipStream_t st);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,

@@ -73,9 +73,20 @@ UFS_DEV bool ecdsa_xcheck(const jac& R, const u32 r[8]) {
     return ok;
 }
 
+// Public key of an item from a pre-decompressed points array (y all-ones
+// bits marks an invalid key).
+UFS_DEV bool load_key_point(aff& P, const aff* pts, u32 i) {
+    load_aff(P, pts + i);
+    u32 m = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m &= P.y.v[k];
+    return m != 0xFFFFFFFFu;
+}
+
 // One ECDSA verification of item i of batch d; returns the verdict.
-// `slot` is the lane's private table area (kSlotEntries affine entries).
-UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot) {
+// `slot` is the lane's private table area (kSlotEntries affine entries);
+// `pts`, when not null, holds the batch's decompressed public keys.
+UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts) {
     u32 z_raw[8], r_raw[8], s_raw[8], x_raw[8];
     load_be32_aligned(z_raw, d.msgs + 32ull * i);
     load_be32_aligned(r_raw, d.sigs + 64ull * i);
@@ -90,7 +101,11 @@ UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* s
     ok &= !sc_is_high(s);
     ok &= (prefix == 2u) | (prefix == 3u);
     aff P;
-    ok &= lift_x(P, x_raw, prefix & 1u);
+    if (pts) {
+        ok &= load_key_point(P, pts, i);
+    } else {
+        ok &= lift_x(P, x_raw, prefix & 1u);
+    }
     if (!ok) {
         // keep the lane on the common path with harmless values
         set_generator(P);
@@ -113,7 +128,8 @@ UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* s
 }
 
 // One BIP-340 verification of item i of batch d.
-UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot) {
+UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot,
+                                 const aff* pts) {
     u32 m_w[8], px_w[8], r_w[8], s_w[8];
     load_be_words(m_w, d.msgs + 32ull * i);
     load_be_words(px_w, d.pubs + 32ull * i);
@@ -125,7 +141,11 @@ UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], co
     u256_from_be_words(ss, s_w);
     bool ok = u256_lt_p(rr) && !u256_is_zero(ss) && u256_lt_n(ss);
     aff P;
-    ok &= lift_x(P, px, 0u);
+    if (pts) {
+        ok &= load_key_point(P, pts, i);
+    } else {
+        ok &= lift_x(P, px, 0u);
+    }
     sc s;
 #pragma unroll
     for (int k = 0; k < 8; ++k) s.v[k] = ss[k];
@@ -195,19 +215,23 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         if (t >= ntiles) break;
         BatchDesc d;
         u32 first;
+        const aff* pts;
         if (list) {
             const TileDesc tl = a.tiles[t];
             d = a.descs[tl.batch];
             first = tl.first;
+            pts = a.batch_pts ? reinterpret_cast<const aff*>(a.batch_pts[tl.batch]) : nullptr;
         } else {
             d = a.single;
             first = t * kTileItems;
+            pts = reinterpret_cast<const aff*>(a.single_pts);
         }
         t += nwaves;
         const u32 i = first + lane;
         if (i >= d.count) continue;
         const u32 kind = KIND == 2 ? d.kind : (u32)KIND;
-        const bool v = kind == 0 ? ecdsa_verify_item(d, i, G, slot) : schnorr_verify_item(d, i, mid, G, slot);
+        const bool v = kind == 0 ? ecdsa_verify_item(d, i, G, slot, pts)
+                                 : schnorr_verify_item(d, i, mid, G, slot, pts);
         if (a.collect) {
             if (v) d.out[i] = 0;
         } else {

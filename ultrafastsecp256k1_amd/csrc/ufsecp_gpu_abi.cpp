@@ -14,6 +14,7 @@
 #include "../../include/ufsecp/ufsecp_gpu_ext.h"
 #include "backend.hpp"
 #include "msm.hpp"
+#include "server.hpp"
 
 using ufs::GpuError;
 using ufs::RocmBackend;
@@ -44,6 +45,10 @@ struct ufsecp_gpu_ctx {
     std::unique_ptr<ufs::MsmEngine> msm;
     uint32_t backend_id;
     uint32_t device_index;
+};
+
+struct ufsecp_gpu_server {
+    std::unique_ptr<ufs::VerifyServer> impl;
 };
 
 #define UFSECP_GPU_CATCH                                           \
@@ -418,6 +423,100 @@ ufsecp_error_t ufsecp_gpu_ext_selftest(ufsecp_gpu_ctx* ctx, int op, const uint32
         return to_abi(ctx->backend->selftest(op, a, b, out, count));
     }
     UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** h_ptr_out) {
+    if (!ctx || !h_ptr_out) return UFSECP_ERR_NULL_ARG;
+    *h_ptr_out = nullptr;
+    if (bytes == 0) return UFSECP_ERR_BAD_INPUT;
+    if (hipHostMalloc(h_ptr_out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *h_ptr_out = nullptr;
+        return UFSECP_ERR_GPU_MEMORY;
+    }
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_host_free(ufsecp_gpu_ctx* ctx, void* h_ptr) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (!h_ptr) return UFSECP_OK;
+    return hipHostFree(h_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_server_create(ufsecp_gpu_ctx* ctx, const ufsecp_gpu_server_config_t* cfg,
+                                            ufsecp_gpu_server** server_out) {
+    if (!ctx || !server_out) return UFSECP_ERR_NULL_ARG;
+    *server_out = nullptr;
+    try {
+        if (!ctx->backend->is_ready()) return UFSECP_ERR_GPU_DEVICE;
+        ufs::ServerConfig c;
+        if (cfg) {
+            if (cfg->hbm_budget_bytes) c.hbm_budget = (size_t)cfg->hbm_budget_bytes;
+            if (cfg->inflight_groups) c.groups = cfg->inflight_groups;
+            c.cache_log2 = cfg->key_cache_log2;
+            if (cfg->max_group_items) c.max_group_items = cfg->max_group_items;
+            if (c.groups > 4 || c.cache_log2 > 26 || c.max_group_items > (1u << 24)) return UFSECP_ERR_BAD_INPUT;
+        }
+        auto s = std::make_unique<ufsecp_gpu_server>();
+        s->impl = std::make_unique<ufs::VerifyServer>(ctx->backend.get());
+        const GpuError e = s->impl->start(c);
+        if (e != GpuError::Ok) return to_abi(e);
+        *server_out = s.release();
+        return UFSECP_OK;
+    }
+    UFSECP_GPU_CATCH
+}
+
+void ufsecp_gpu_ext_server_destroy(ufsecp_gpu_server* server) { delete server; }
+
+ufsecp_error_t ufsecp_gpu_ext_server_submit(ufsecp_gpu_server* server, uint32_t kind, const uint8_t* msg_hashes32,
+                                            const uint8_t* pubkeys, const uint8_t* sigs64, size_t count,
+                                            uint8_t* out_results, uint64_t* request_id) {
+    if (!server || !request_id) return UFSECP_ERR_NULL_ARG;
+    *request_id = 0;
+    if (kind > 1 || count == 0) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !pubkeys || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(server->impl->submit(kind, msg_hashes32, pubkeys, sigs64, count, out_results, request_id));
+    }
+    UFSECP_GPU_CATCH
+}
+
+int ufsecp_gpu_ext_server_poll(ufsecp_gpu_server* server, uint64_t request_id) {
+    if (!server) return -1;
+    return server->impl->poll(request_id);
+}
+
+ufsecp_error_t ufsecp_gpu_ext_server_wait(ufsecp_gpu_server* server, uint64_t request_id) {
+    if (!server) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(server->impl->wait(request_id));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_server_drain(ufsecp_gpu_server* server) {
+    if (!server) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(server->impl->drain());
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_server_stats(ufsecp_gpu_server* server, ufsecp_gpu_server_stats_t* out, int reset) {
+    if (!server || !out) return UFSECP_ERR_NULL_ARG;
+    ufs::ServerStats s;
+    server->impl->stats(s, reset != 0);
+    out->requests_completed = s.requests_completed;
+    out->items_completed = s.items_completed;
+    out->groups_launched = s.groups_launched;
+    out->key_lookups = s.key_lookups;
+    out->key_cache_hits = s.key_cache_hits;
+    out->latency_p50_us = s.latency_p50_us;
+    out->latency_p99_us = s.latency_p99_us;
+    out->latency_mean_us = s.latency_mean_us;
+    out->latency_max_us = s.latency_max_us;
+    out->busy_seconds = s.busy_seconds;
+    return UFSECP_OK;
 }
 
 }  // extern "C"

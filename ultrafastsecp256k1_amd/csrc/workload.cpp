@@ -456,11 +456,62 @@ void parallel_chunks(u64 n, int threads, F&& f) {
     for (auto& t : ts) t.join();
 }
 
+// Key of an item.  Plain workloads: the item stream's first key() draw.
+// Serving workloads (pool_size > 0): a first draw u picks, with probability
+// pool_permille / 1000, key number (u >> 20) % pool_size of the pool (pool key
+// j = the first key() draw of stream (pool_seed, j)); otherwise the item
+// stream's next key() draw.
+struct KeyPool {
+    u64 seed = 0;
+    u64 size = 0;
+    u32 permille = 0;
+    U256 key(Rng& r) const {
+        if (size == 0) return r.key();
+        const u64 u = r.next();
+        if (u % 1000 < permille) {
+            Rng p(seed, (u >> 20) % size);
+            return p.key();
+        }
+        return r.key();
+    }
+};
+
+int wl_ecdsa(const KeyPool& kp, u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
+             int threads);
+int wl_schnorr(const KeyPool& kp, u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
+               int threads);
+
 }  // namespace
 
 // ECDSA items [first, first+n): msgs n*32, pubs n*33, sigs n*64, corrupted n.
 WL_API int ufsecp_wl_ecdsa(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
                            int threads) {
+    return wl_ecdsa(KeyPool{}, seed, first, n, msgs, pubs, sigs, corrupted, threads);
+}
+
+// BIP-340 items: msgs n*32, pubs (x-only) n*32, sigs n*64, corrupted n.
+WL_API int ufsecp_wl_schnorr(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
+                             int threads) {
+    return wl_schnorr(KeyPool{}, seed, first, n, msgs, pubs, sigs, corrupted, threads);
+}
+
+// Serving workloads: as above, with a share of the keys drawn from a pool.
+WL_API int ufsecp_wl_ecdsa_pool(u64 seed, u64 first, u64 n, u64 pool_seed, u64 pool_size, u32 pool_permille,
+                                u8* msgs, u8* pubs, u8* sigs, u8* corrupted, int threads) {
+    return wl_ecdsa(KeyPool{pool_seed, pool_size, pool_permille}, seed, first, n, msgs, pubs, sigs, corrupted,
+                    threads);
+}
+
+WL_API int ufsecp_wl_schnorr_pool(u64 seed, u64 first, u64 n, u64 pool_seed, u64 pool_size, u32 pool_permille,
+                                  u8* msgs, u8* pubs, u8* sigs, u8* corrupted, int threads) {
+    return wl_schnorr(KeyPool{pool_seed, pool_size, pool_permille}, seed, first, n, msgs, pubs, sigs, corrupted,
+                      threads);
+}
+
+namespace {
+
+int wl_ecdsa(const KeyPool& kp, u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
+             int threads) {
     if (n == 0) return 0;
     parallel_chunks(n, threads, [&](u64 b, u64 e) {
         const size_t m = e - b;
@@ -469,7 +520,7 @@ WL_API int ufsecp_wl_ecdsa(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* s
         std::vector<Jac> Q(m), R(m);
         for (size_t j = 0; j < m; ++j) {
             Rng r(seed, first + b + j);
-            d[j] = r.key();
+            d[j] = kp.key(r);
             u8* msg = msgs + 32 * (b + j);
             r.next256(msg);
             cw[j] = r.next();
@@ -506,9 +557,8 @@ WL_API int ufsecp_wl_ecdsa(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* s
     return 0;
 }
 
-// BIP-340 items: msgs n*32, pubs (x-only) n*32, sigs n*64, corrupted n.
-WL_API int ufsecp_wl_schnorr(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
-                             int threads) {
+int wl_schnorr(const KeyPool& kp, u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8* sigs, u8* corrupted,
+               int threads) {
     if (n == 0) return 0;
     parallel_chunks(n, threads, [&](u64 b, u64 e) {
         const size_t m = e - b;
@@ -518,7 +568,7 @@ WL_API int ufsecp_wl_schnorr(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8*
         std::vector<std::array<u8, 32>> aux(m);
         for (size_t j = 0; j < m; ++j) {
             Rng r(seed, first + b + j);
-            d[j] = r.key();
+            d[j] = kp.key(r);
             r.next256(msgs + 32 * (b + j));
             r.next256(aux[j].data());
             cw[j] = r.next();
@@ -558,6 +608,8 @@ WL_API int ufsecp_wl_schnorr(u64 seed, u64 first, u64 n, u8* msgs, u8* pubs, u8*
     });
     return 0;
 }
+
+}  // namespace
 
 // MSM terms: scalars n*32 (uniform 256-bit reduced mod n), points n*33 = k_i*G
 // compressed, ks n*32 (the k_i, for the closed form).

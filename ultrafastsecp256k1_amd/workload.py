@@ -123,6 +123,11 @@ def _lib() -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, ctypes.c_int]
+        for name in ("ufsecp_wl_ecdsa_pool", "ufsecp_wl_schnorr_pool"):
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                           ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_int]
         lib.ufsecp_wl_msm.restype = ctypes.c_int
         lib.ufsecp_wl_msm.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, ctypes.c_int]
         _WL = lib
@@ -159,6 +164,26 @@ def generate(kind: str, n: int, seed: Optional[int] = None, first: int = 0, thre
     corr = np.zeros(n, np.uint8)
     fn = lib.ufsecp_wl_ecdsa if kind == "ecdsa" else lib.ufsecp_wl_schnorr
     rc = fn(seed, first, n, _ptr(msgs), _ptr(pubs), _ptr(sigs), _ptr(corr), threads)
+    if rc != 0:
+        raise RuntimeError(f"workload generation failed ({rc})")
+    return VerifyBatch(kind, msgs, pubs, sigs, corr)
+
+
+def generate_pool(kind: str, n: int, seed: int, pool_seed: int, pool_size: int = 1 << 16,
+                  pool_permille: int = 200, first: int = 0, threads: int = 0) -> VerifyBatch:
+    """Serving workload items: as generate(), except that each item's key is,
+    with probability pool_permille/1000, one of pool_size keys of a seeded pool
+    (first draw u of the item stream: u % 1000 < pool_permille selects pool key
+    (u >> 20) % pool_size, pool key j being stream (pool_seed, j)'s first key()
+    draw); otherwise the item stream's next key() draw."""
+    lib = _lib()
+    msgs = np.zeros((n, 32), np.uint8)
+    pubs = np.zeros((n, 33 if kind == "ecdsa" else 32), np.uint8)
+    sigs = np.zeros((n, 64), np.uint8)
+    corr = np.zeros(n, np.uint8)
+    fn = lib.ufsecp_wl_ecdsa_pool if kind == "ecdsa" else lib.ufsecp_wl_schnorr_pool
+    rc = fn(seed, first, n, pool_seed, pool_size, pool_permille, _ptr(msgs), _ptr(pubs), _ptr(sigs), _ptr(corr),
+            threads)
     if rc != 0:
         raise RuntimeError(f"workload generation failed ({rc})")
     return VerifyBatch(kind, msgs, pubs, sigs, corr)

@@ -53,6 +53,27 @@ def profiled_traffic(kind: str, n: int):
     return None, None
 
 
+def bench_serve(seconds: float, ecdsa_items: int, schnorr_items: int) -> dict:
+    """Serving loop under Poisson arrivals (ultrafastsecp256k1_amd/serve_bench):
+    continuous batching vs one launch per request, p50/p99 at 50/80/95% of
+    each server's own sustained peak."""
+    exe = ROOT / "ultrafastsecp256k1_amd" / "serve_bench"
+    cmd = [str(exe), "--seconds", str(seconds), "--ecdsa-items", str(ecdsa_items),
+           "--schnorr-items", str(schnorr_items)]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    for line in res.stderr.splitlines():
+        log(line)
+    if res.returncode not in (0, 1):
+        raise RuntimeError(f"serve_bench failed ({res.returncode}): {res.stderr[-2000:]}")
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    b = out["batched"]
+    for r in b["loads"]:
+        r["key_cache_hit_rate"] = r["key_cache_hits"] / max(1, r["key_lookups"])
+    if out.get("plain"):
+        out["peak_ratio_batched_vs_plain"] = b["peak_sigs_per_s"] / max(1.0, out["plain"]["peak_sigs_per_s"])
+    return out
+
+
 def measure_valu_peak() -> dict:
     """Run the v_mad_u64_u32 throughput probe; returns its JSON records."""
     exe = ROOT / "ultrafastsecp256k1_amd" / "valu_peak"
@@ -226,8 +247,11 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-items", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="ecdsa,mix,msm",
-                    help="extra measurements after the headline: ecdsa, mix (uneven batches), msm")
+    ap.add_argument("--legs", default="ecdsa,mix,msm,serve",
+                    help="extra measurements after the headline: ecdsa, mix (uneven batches), msm, serve")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-items", default="4194304,2097152",
+                    help="ECDSA,BIP-340 items pre-generated for the serving leg")
     ap.add_argument("--mix-batches", type=int, default=10000)
     ap.add_argument("--mix-plain-batches", type=int, default=500)
     ap.add_argument("--msm-n", type=int, default=1 << 20)
@@ -275,14 +299,23 @@ def main() -> int:
         log(f"[bench] mix: {legs['mix']}")
     if "msm" in want:
         legs["msm"] = bench_msm(gpu, args.msm_n, [int(c) for c in args.msm_windows.split(",")], 3)
+    if "serve" in want:
+        ne, ns = (int(x) for x in args.serve_items.split(","))
+        legs["serve"] = bench_serve(args.serve_seconds, ne, ns)
     parity = head["parity"] and all(v.get("parity", True) for v in legs.values())
+    latency = None
+    if "serve" in legs:
+        top = legs["serve"]["batched"]["loads"][-1]
+        latency = {"at_load_of_peak": top["frac"], "sustained_sigs_per_s": top["achieved_sigs_per_s"],
+                   "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "source": "legs.serve (continuous batching)"}
     out = {
         "metric": METRIC, "value": head["sigs_per_s"], "unit": "sigs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"{kind} batch verify, {n} signatures in one launch, 1% corrupted",
                    "batch": n, "kind": kind},
-        "parity": parity, "roofline": roofline, "cpu_baseline": cpu, "legs": legs, "ctx_create_ms": round(ctx_ms, 1),
+        "parity": parity, "roofline": roofline, "cpu_baseline": cpu, "latency": latency, "legs": legs,
+        "ctx_create_ms": round(ctx_ms, 1),
         "valu_peak": {k: v for k, v in peak.items() if k != "device"},
         "device": peak["device"],
     }

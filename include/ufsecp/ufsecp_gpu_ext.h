@@ -99,8 +99,9 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_synchronize(ufsecp_gpu_ctx* ctx);
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_selftest(ufsecp_gpu_ctx* ctx, int op, const uint32_t* a, const uint32_t* b,
                                                   uint32_t* out, size_t count);
 
-/* Pinned (page-locked) host memory: inputs submitted from such buffers are
- * copied at full DMA rate.  Not a device allocation. */
+/* Pinned (page-locked) host memory, mapped into the device's address space at
+ * the same address: kernels can read it in place (see
+ * ufsecp_gpu_ext_server_submit_mapped).  Not a device allocation. */
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** h_ptr_out);
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_host_free(ufsecp_gpu_ctx* ctx, void* h_ptr);
 
@@ -151,6 +152,13 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_submit(ufsecp_gpu_server* server
                                                        const uint8_t* msg_hashes32, const uint8_t* pubkeys,
                                                        const uint8_t* sigs64, size_t count, uint8_t* out_results,
                                                        uint64_t* request_id);
+/* As ufsecp_gpu_ext_server_submit, for inputs the GPU can read in place:
+ * buffers from ufsecp_gpu_ext_host_alloc or device memory (msgs and sigs, and
+ * x-only pubkeys, 16-byte aligned).  No staging copy is made. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_submit_mapped(ufsecp_gpu_server* server, uint32_t kind,
+                                                              const uint8_t* msg_hashes32, const uint8_t* pubkeys,
+                                                              const uint8_t* sigs64, size_t count,
+                                                              uint8_t* out_results, uint64_t* request_id);
 /* 1: done (the id is then forgotten), 0: pending, -1: unknown id. */
 UFSECP_API int ufsecp_gpu_ext_server_poll(ufsecp_gpu_server* server, uint64_t request_id);
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_wait(ufsecp_gpu_server* server, uint64_t request_id);

@@ -146,3 +146,20 @@ def test_server_edge_cases(gpu, golden, kind):
             assert np.array_equal(outs[1], _plain(gpu, k, mm, pp, ss))
             for i in range(len(cases)):
                 assert int(outs[2 + i][0]) == int(exp[i]), cases[i]["name"]
+
+
+def test_server_mapped_submission(gpu):
+    """Inputs in pinned, device-mapped host memory are read in place."""
+    reqs = _requests(n_ecdsa=3000, n_schnorr=1500, seed=0x44)
+    pinned = []
+    for kind, m, p, s, exp in reqs:
+        pm, pp, ps = gpu.host_empty(m.shape), gpu.host_empty(p.shape), gpu.host_empty(s.shape)
+        pm[:], pp[:], ps[:] = m, p, s
+        pinned.append((kind, pm, pp, ps, exp))
+    outs = [np.full(len(r[1]), 7, np.uint8) for r in pinned]
+    with VerifyServer(gpu, key_cache_log2=12) as srv:
+        for (kind, m, p, s, _), out in zip(pinned, outs):
+            srv.submit(kind, m, p, s, out, mapped=True)
+        srv.drain()
+    for (kind, m, p, s, exp), out in zip(pinned, outs):
+        assert np.array_equal(out, exp)

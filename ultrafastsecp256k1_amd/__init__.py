@@ -47,6 +47,7 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_event_create", "ufsecp_gpu_ext_event_record", "ufsecp_gpu_ext_event_elapsed",
     "ufsecp_gpu_ext_event_destroy", "ufsecp_gpu_ext_host_alloc", "ufsecp_gpu_ext_host_free",
     "ufsecp_gpu_ext_server_create", "ufsecp_gpu_ext_server_destroy", "ufsecp_gpu_ext_server_submit",
+    "ufsecp_gpu_ext_server_submit_mapped",
     "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
     "ufsecp_gpu_ext_server_stats",
 )
@@ -148,6 +149,8 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_server_destroy": (None, [vp]),
         "ufsecp_gpu_ext_server_submit": (c_int, [vp, c_uint32, vp, vp, vp, c_size_t, vp,
                                                  POINTER(ctypes.c_uint64)]),
+        "ufsecp_gpu_ext_server_submit_mapped": (c_int, [vp, c_uint32, vp, vp, vp, c_size_t, vp,
+                                                        POINTER(ctypes.c_uint64)]),
         "ufsecp_gpu_ext_server_poll": (c_int, [vp, ctypes.c_uint64]),
         "ufsecp_gpu_ext_server_wait": (c_int, [vp, ctypes.c_uint64]),
         "ufsecp_gpu_ext_server_drain": (c_int, [vp]),
@@ -324,6 +327,23 @@ class UfsecpGpu:
     def event(self) -> "Event":
         return Event(self)
 
+    def host_empty(self, shape, dtype="uint8"):
+        """numpy array in pinned host memory that the GPU reads in place
+        (ufsecp_gpu_ext_host_alloc); freed with the array's last reference."""
+        import numpy as np
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = c_void_p()
+        rc = self._lib.ufsecp_gpu_ext_host_alloc(self._ctx, max(1, n), byref(p))
+        if rc != UFSECP_OK:
+            raise UfsecpError("ufsecp_gpu_ext_host_alloc", rc)
+        raw = (ctypes.c_uint8 * max(1, n)).from_address(p.value)
+        arr = np.frombuffer(raw, dtype=np.uint8, count=n).view(dt).reshape(shape)
+        lib, ctx, addr = self._lib, self._ctx, p.value
+        import weakref
+        weakref.finalize(raw, lambda: lib.ufsecp_gpu_ext_host_free(ctx, c_void_p(addr)))
+        return arr
+
     def synchronize(self) -> None:
         self._check("ufsecp_gpu_ext_synchronize", self._lib.ufsecp_gpu_ext_synchronize(self._ctx))
 
@@ -359,11 +379,13 @@ class VerifyServer:
         self._h = h
         self._keep = {}
 
-    def submit(self, kind: int, msgs, pubkeys, sigs, out) -> int:
+    def submit(self, kind: int, msgs, pubkeys, sigs, out, mapped: bool = False) -> int:
+        """mapped=True: the inputs live in memory from UfsecpGpu.host_empty and
+        are read in place by the GPU (no staging copy)."""
         n = _nitems(msgs, 32)
         rid = ctypes.c_uint64()
-        rc = self._lib.ufsecp_gpu_ext_server_submit(self._h, kind, _buf(msgs), _buf(pubkeys), _buf(sigs), n,
-                                                    _buf(out), byref(rid))
+        fn = self._lib.ufsecp_gpu_ext_server_submit_mapped if mapped else self._lib.ufsecp_gpu_ext_server_submit
+        rc = fn(self._h, kind, _buf(msgs), _buf(pubkeys), _buf(sigs), n, _buf(out), byref(rid))
         if rc != UFSECP_OK:
             raise UfsecpError("ufsecp_gpu_ext_server_submit", rc)
         self._keep[rid.value] = (msgs, pubkeys, sigs, out)

@@ -83,14 +83,30 @@ def build_workload() -> Path:
     return WORKLOAD_LIB
 
 
+SERVE_BENCH = PKG / "serve_bench"
+
+
+def build_serve_bench() -> Path:
+    """Serving benchmark executable over the two libraries (built first)."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = SERVE_BENCH.with_suffix(".tmp")
+    _run([gxx, "-O2", "-std=c++17", "-pthread", f"-I{ROOT / 'include'}", str(CSRC / "serve_bench.cpp"),
+          f"-L{PKG}", "-lufsecp_gpu", "-lufsecp_workload", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(tmp)])
+    os.replace(tmp, SERVE_BENCH)
+    return SERVE_BENCH
+
+
 def build_all() -> None:
     with cf.ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_gpu), ex.submit(build_workload), ex.submit(build_valu_peak)]
         for f in futs:
             f.result()
+    build_serve_bench()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"gpu": build_gpu, "workload": build_workload, "valu": build_valu_peak, "all": build_all}[what]()
+    {"gpu": build_gpu, "workload": build_workload, "valu": build_valu_peak, "serve": build_serve_bench,
+     "all": build_all}[what]()
     print("built", what)

@@ -163,13 +163,16 @@ VerifyServer::~VerifyServer() {
 }
 
 bool VerifyServer::try_admit_locked(Request* r) {
-    const size_t in_len = in_layout(r->kind, r->count).len;
+    const size_t in_len = r->mapped ? 0 : in_layout(r->kind, r->count).len;
     const size_t pt_len = align_up((size_t)64 * r->count, kAlign);
     const size_t out_len = align_up(r->count, kAlign);
     size_t c_in = 0, c_pt = 0, c_out = 0;
     FifoRing in_try = in_ring_, pt_try = pt_ring_, out_try = out_ring_;
-    const size_t o_in = in_try.alloc(in_len, c_in);
-    if (o_in == kNone) return false;
+    size_t o_in = 0;
+    if (!r->mapped) {
+        o_in = in_try.alloc(in_len, c_in);
+        if (o_in == kNone) return false;
+    }
     const size_t o_pt = pt_try.alloc(pt_len, c_pt);
     if (o_pt == kNone) return false;
     const size_t o_out = out_try.alloc(out_len, c_out);
@@ -184,12 +187,14 @@ bool VerifyServer::try_admit_locked(Request* r) {
     r->pt_charge = c_pt;
     r->out_off = o_out;
     r->out_charge = c_out;
+    if (r->mapped) r->staged = true;
     admitted_.push_back(r);
     to_launch_.push_back(r);
     return true;
 }
 
 void VerifyServer::stage_inputs(Request* r) {
+    if (r->mapped) return;
     const InLayout l = in_layout(r->kind, r->count);
     uint8_t* base = h_in_ + r->in_off;
     std::memcpy(base + l.msgs, r->msgs, 32ull * r->count);
@@ -198,11 +203,15 @@ void VerifyServer::stage_inputs(Request* r) {
 }
 
 GpuError VerifyServer::submit(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
-                              size_t count, uint8_t* out, uint64_t* id) {
+                              size_t count, uint8_t* out, bool mapped, uint64_t* id) {
     if (err_.load()) return error();
     if (kind > 1 || count == 0 || count > cfg_.max_group_items) return GpuError::BadInput;
     if (in_layout(kind, count).len > in_ring_.cap) return GpuError::BadInput;
+    // in-place reads use 16-byte loads of the 32-byte fields
+    if (mapped && ((((uintptr_t)msgs | (uintptr_t)sigs) & 15) || (kind == 1 && ((uintptr_t)pubs & 15))))
+        return GpuError::BadInput;
     Request* r = new Request();
+    r->mapped = mapped;
     r->kind = kind;
     r->count = (uint32_t)count;
     r->msgs = msgs;
@@ -220,33 +229,41 @@ GpuError VerifyServer::submit(uint32_t kind, const uint8_t* msgs, const uint8_t*
         if (!admitted) pending_.push_back(r);
     }
     *id = r->id;
-    if (admitted) {
+    if (admitted && !mapped) {
         stage_inputs(r);
         {
             std::lock_guard<std::mutex> lk(mu_);
             r->staged = true;
         }
-        cv_sched_.notify_one();
     }
+    if (admitted) cv_sched_.notify_one();
     return GpuError::Ok;
 }
 
 GpuError VerifyServer::launch_group(Group& g, std::vector<Request*>& batch) {
     hipStream_t st = g.stream;
-    // inputs: the group's input blocks are contiguous in ring order except
-    // for at most one wrap
+    // inputs: the staged input blocks of a group are contiguous in ring order
+    // except for at most one wrap; mapped requests are read in place
     hipError_t e = hipSuccess;
-    size_t run_start = batch[0]->in_off, run_end = run_start;
+    bool have_run = false;
+    size_t run_start = 0, run_end = 0;
     for (Request* r : batch) {
-        if (r->in_off != run_end) {
+        if (r->mapped) continue;
+        if (have_run && r->in_off != run_end) {
             e = hipMemcpyAsync(d_in_ + run_start, h_in_ + run_start, run_end - run_start, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return GpuError::Memory;
+            have_run = false;
+        }
+        if (!have_run) {
             run_start = r->in_off;
+            have_run = true;
         }
         run_end = r->in_off + r->in_len;
     }
-    e = hipMemcpyAsync(d_in_ + run_start, h_in_ + run_start, run_end - run_start, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return GpuError::Memory;
+    if (have_run) {
+        e = hipMemcpyAsync(d_in_ + run_start, h_in_ + run_start, run_end - run_start, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return GpuError::Memory;
+    }
 
     // tables: descriptors | tiles | key-point pointers | tile count
     const size_t nb = batch.size();
@@ -256,9 +273,14 @@ GpuError VerifyServer::launch_group(Group& g, std::vector<Request*>& batch) {
     size_t nt = 0;
     for (size_t b = 0; b < nb; ++b) {
         const Request* r = batch[b];
-        const InLayout l = in_layout(r->kind, r->count);
-        const uint8_t* base = d_in_ + r->in_off;
-        hd[b] = BatchDesc{r->kind, r->count, base + l.msgs, base + l.pubs, base + l.sigs, d_out_view_ + r->out_off};
+        if (r->mapped) {
+            hd[b] = BatchDesc{r->kind, r->count, r->msgs, r->pubs, r->sigs, d_out_view_ + r->out_off};
+        } else {
+            const InLayout l = in_layout(r->kind, r->count);
+            const uint8_t* base = d_in_ + r->in_off;
+            hd[b] = BatchDesc{r->kind, r->count, base + l.msgs, base + l.pubs, base + l.sigs,
+                              d_out_view_ + r->out_off};
+        }
         for (uint32_t f = 0; f < r->count; f += kTileItems) ht[nt++] = TileDesc{(uint32_t)b, f};
     }
     const size_t off_pts = off_tiles + align_up(nt * sizeof(TileDesc), kAlign);

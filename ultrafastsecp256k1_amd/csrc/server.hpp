@@ -68,8 +68,10 @@ public:
     explicit VerifyServer(RocmBackend* be) : be_(be) {}
     ~VerifyServer();
     GpuError start(const ServerConfig& cfg);
+    // mapped: inputs are device-accessible (pinned host memory from the
+    // library's allocator, or device memory) and are read in place.
     GpuError submit(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs, size_t count,
-                    uint8_t* out, uint64_t* id);
+                    uint8_t* out, bool mapped, uint64_t* id);
     int poll(uint64_t id);   // 1 done (forgotten), 0 pending, -1 unknown
     GpuError wait(uint64_t id);
     GpuError drain();
@@ -86,6 +88,7 @@ private:
         size_t pt_off, pt_charge;            // device ring of decompressed keys
         size_t out_off, out_charge;          // pinned verdict ring
         double t_submit;
+        bool mapped = false;                  // inputs read in place, no input ring
         bool staged = false, done = false;
         bool released = false, observed = false;   // freed when both
     };

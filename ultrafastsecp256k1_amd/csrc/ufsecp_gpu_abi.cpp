@@ -429,9 +429,16 @@ ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void
     if (!ctx || !h_ptr_out) return UFSECP_ERR_NULL_ARG;
     *h_ptr_out = nullptr;
     if (bytes == 0) return UFSECP_ERR_BAD_INPUT;
-    if (hipHostMalloc(h_ptr_out, bytes, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(h_ptr_out, bytes, hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
         *h_ptr_out = nullptr;
         return UFSECP_ERR_GPU_MEMORY;
+    }
+    // kernels read this memory through the host address itself
+    void* dptr = nullptr;
+    if (hipHostGetDevicePointer(&dptr, *h_ptr_out, 0) != hipSuccess || dptr != *h_ptr_out) {
+        (void)hipHostFree(*h_ptr_out);
+        *h_ptr_out = nullptr;
+        return UFSECP_ERR_GPU_UNSUPPORTED;
     }
     return UFSECP_OK;
 }
@@ -476,7 +483,23 @@ ufsecp_error_t ufsecp_gpu_ext_server_submit(ufsecp_gpu_server* server, uint32_t 
     if (kind > 1 || count == 0) return UFSECP_ERR_BAD_INPUT;
     if (!msg_hashes32 || !pubkeys || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
     try {
-        return to_abi(server->impl->submit(kind, msg_hashes32, pubkeys, sigs64, count, out_results, request_id));
+        return to_abi(
+            server->impl->submit(kind, msg_hashes32, pubkeys, sigs64, count, out_results, false, request_id));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_server_submit_mapped(ufsecp_gpu_server* server, uint32_t kind,
+                                                   const uint8_t* msg_hashes32, const uint8_t* pubkeys,
+                                                   const uint8_t* sigs64, size_t count, uint8_t* out_results,
+                                                   uint64_t* request_id) {
+    if (!server || !request_id) return UFSECP_ERR_NULL_ARG;
+    *request_id = 0;
+    if (kind > 1 || count == 0) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !pubkeys || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(
+            server->impl->submit(kind, msg_hashes32, pubkeys, sigs64, count, out_results, true, request_id));
     }
     UFSECP_GPU_CATCH
 }

@@ -44,58 +44,144 @@ UFS_DEV int msm_digit(const u32 k[8], int w, int c, u32& carry) {
     return (int)raw;
 }
 
+// Counting sort of the (window, |digit|) bucket ids (counted in
+// k_msm_prepare, placed by k_msm_scatter).  Windows below the top
+// one have 2^(c-1) uniformly used buckets and count with plain global atomics.
+// The top window has only 2^top_bits buckets (top_bits = 257 - c(W-1)) that
+// receive a large share of all terms; their counters share a few cache lines,
+// so per-term global atomics on them serialise (measured 2-6 ms at 2^20
+// terms).  When top_bits <= 8 the top window is counted in LDS per block and
+// each block reserves its range with one global atomic per bucket.
+constexpr int kTermsPerThread = 8;
+constexpr int kTopLdsBits = 8;
+
+UFS_DEV int msm_top_bits(int c, int W) { return 257 - c * (W - 1); }
+
+// Parse + decompress every term (scalar < n, strict SEC1 point; the first
+// failing term decides the error, reported as 2i or 2i+1 through atomicMin)
+// and count its bucket ids.  The counting atomics overlap the square roots.
+constexpr int kPrepTermsPerThread = 4;
+
 __global__ void __launch_bounds__(kMsmBlock)
 k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33, u32 n, aff* __restrict__ pts,
               sc* __restrict__ scl, u32* __restrict__ counts, int c, int W, u32* __restrict__ err) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    u32 k[8];
-    msm_load_scalar(k, scalars32 + 32ull * i);
-    const bool sok = u256_lt_n(k);
-    const u8* pp = points33 + 33ull * i;
-    const u32 prefix = pp[0];
-    u32 x[8];
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-        const u8* q = pp + 1 + 28 - 4 * l;
-        x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
-    }
-    aff P;
-    bool pok = lift_x(P, x, prefix & 1u);
-    pok &= (prefix == 2u) | (prefix == 3u);
-    if (!sok) {
-        atomicMin(err, 2u * i);
-    } else if (!pok) {
-        atomicMin(err, 2u * i + 1u);
-    }
-    pts[i] = P;
-    sc s;
-#pragma unroll
-    for (int l = 0; l < 8; ++l) s.v[l] = k[l];
-    scl[i] = s;
+    __shared__ u32 tcnt[1 << kTopLdsBits];
     const u32 B = 1u << (c - 1);
-    u32 carry = 0;
-    for (int w = 0; w < W; ++w) {
-        const int d = msm_digit(k, w, c, carry);
-        if (d != 0) atomicAdd(&counts[w * B + (u32)(d < 0 ? -d : d) - 1u], 1u);
+    const int tb = msm_top_bits(c, W);
+    const bool lds = tb <= kTopLdsBits;
+    const u32 nkeys = 1u << (tb < kTopLdsBits ? tb : kTopLdsBits);
+    if (lds) {
+        for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) tcnt[t] = 0;
+        __syncthreads();
+    }
+    const size_t step = (size_t)gridDim.x * blockDim.x * kPrepTermsPerThread;
+    for (size_t base = (size_t)blockIdx.x * blockDim.x * kPrepTermsPerThread; base < n; base += step) {
+        for (int j = 0; j < kPrepTermsPerThread; ++j) {
+            const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
+            if (i >= n) break;
+            u32 k[8];
+            msm_load_scalar(k, scalars32 + 32ull * i);
+            const bool sok = u256_lt_n(k);
+            const u8* pp = points33 + 33ull * i;
+            const u32 prefix = pp[0];
+            u32 x[8];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                const u8* q = pp + 1 + 28 - 4 * l;
+                x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
+            }
+            aff P;
+            bool pok = lift_x(P, x, prefix & 1u);
+            pok &= (prefix == 2u) | (prefix == 3u);
+            if (!sok) {
+                atomicMin(err, 2u * (u32)i);
+            } else if (!pok) {
+                atomicMin(err, 2u * (u32)i + 1u);
+            }
+            pts[i] = P;
+            sc s;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) s.v[l] = k[l];
+            scl[i] = s;
+            u32 carry = 0;
+            for (int w = 0; w < W; ++w) {
+                const int d = msm_digit(k, w, c, carry);
+                if (d == 0) continue;
+                const u32 m = (u32)(d < 0 ? -d : d) - 1u;
+                if (lds && w == W - 1) {
+                    atomicAdd(&tcnt[m], 1u);
+                } else {
+                    atomicAdd(&counts[w * B + m], 1u);
+                }
+            }
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x)
+            if (tcnt[t]) atomicAdd(&counts[(W - 1) * B + t], tcnt[t]);
     }
 }
 
+// Scatter of (term | sign) into bucket order; top-window positions come from
+// per-block LDS ranks plus one reserved range per bucket and block.
 __global__ void __launch_bounds__(kMsmBlock)
 k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* __restrict__ entries,
               u32* __restrict__ keys, int c, int W) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const sc s = scl[i];
+    __shared__ u32 tcnt[1 << kTopLdsBits];
+    __shared__ u32 tbase[1 << kTopLdsBits];
     const u32 B = 1u << (c - 1);
-    u32 carry = 0;
-    for (int w = 0; w < W; ++w) {
-        const int d = msm_digit(s.v, w, c, carry);
-        if (d != 0) {
-            const u32 b = w * B + (u32)(d < 0 ? -d : d) - 1u;
-            const u32 pos = atomicAdd(&cursor[b], 1u);
-            entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
-            keys[pos] = b;
+    const int tb = msm_top_bits(c, W);
+    const bool lds = tb <= kTopLdsBits;
+    const u32 nkeys = 1u << (tb < kTopLdsBits ? tb : kTopLdsBits);
+    const size_t step = (size_t)gridDim.x * blockDim.x * kTermsPerThread;
+    for (size_t base = (size_t)blockIdx.x * blockDim.x * kTermsPerThread; base < n; base += step) {
+        if (lds) {
+            for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) tcnt[t] = 0;
+            __syncthreads();
+        }
+        u32 rank[kTermsPerThread];
+        int dtop[kTermsPerThread];
+#pragma unroll
+        for (int j = 0; j < kTermsPerThread; ++j) {
+            dtop[j] = 0;
+            rank[j] = 0;
+            const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
+            if (i >= n) continue;
+            const sc s = scl[i];
+            u32 carry = 0;
+            for (int w = 0; w < W; ++w) {
+                const int d = msm_digit(s.v, w, c, carry);
+                if (d == 0) continue;
+                const u32 b = w * B + (u32)(d < 0 ? -d : d) - 1u;
+                if (lds && w == W - 1) {
+                    dtop[j] = d;
+                    rank[j] = atomicAdd(&tcnt[b - w * B], 1u);
+                } else {
+                    const u32 pos = atomicAdd(&cursor[b], 1u);
+                    entries[pos] = (u32)i | (d < 0 ? 0x80000000u : 0u);
+                    keys[pos] = b;
+                }
+            }
+        }
+        if (lds) {
+            __syncthreads();
+            for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) {
+                const u32 cnt = tcnt[t];
+                tbase[t] = cnt ? atomicAdd(&cursor[(W - 1) * B + t], cnt) : 0u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTermsPerThread; ++j) {
+                const int d = dtop[j];
+                if (d == 0) continue;
+                const u32 m = (u32)(d < 0 ? -d : d) - 1u;
+                const u32 pos = tbase[m] + rank[j];
+                const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
+                entries[pos] = (u32)i | (d < 0 ? 0x80000000u : 0u);
+                keys[pos] = (W - 1) * B + m;
+            }
+            __syncthreads();
         }
     }
 }
@@ -319,15 +405,26 @@ hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
 }
 
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* counts, const MsmPlan& p, uint32_t* err, hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_prepare, dim3(blocks_for(n, kMsmBlock)), dim3(kMsmBlock), 0, st, scalars32, points33,
-                       n, (aff*)pts, (sc*)scl, counts, p.c, p.windows, err);
+                              uint32_t* counts, const MsmPlan& p, uint32_t* err, unsigned max_blocks,
+                              hipStream_t st) {
+    unsigned g = blocks_for(n, (unsigned)kMsmBlock * kPrepTermsPerThread);
+    if (g > max_blocks) g = max_blocks;
+    hipLaunchKernelGGL(k_msm_prepare, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points33, n, (aff*)pts, (sc*)scl,
+                       counts, p.c, p.windows, err);
     return hipGetLastError();
 }
 
+// Grid of the counting-sort kernels: kTermsPerThread terms per lane per
+// pass, at most `max_blocks` blocks (fewer blocks = fewer reservations of
+// the top window's hot counters).
+static unsigned sort_grid(uint32_t n, unsigned max_blocks) {
+    const unsigned b = blocks_for(n, (unsigned)kMsmBlock * kTermsPerThread);
+    return b < max_blocks ? b : max_blocks;
+}
+
 hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
-                              const MsmPlan& p, hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(n, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
+                              const MsmPlan& p, unsigned max_blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_msm_scatter, dim3(sort_grid(n, max_blocks)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
                        cursor, entries, keys, p.c, p.windows);
     return hipGetLastError();
 }
@@ -407,6 +504,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     uint32_t* counts = (uint32_t*)counts_.ptr;
     uint32_t* offsets = (uint32_t*)offsets_.ptr;
+    const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
+    const unsigned sort_blocks = cus * 2, prep_blocks = cus * 4;
     uint32_t* err = (uint32_t*)err_.ptr;
     uint32_t* sel_idx = (uint32_t*)sel_idx_.ptr;
     uint32_t* sel_count = (uint32_t*)sel_count_.ptr;
@@ -422,15 +521,15 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
 
     if ((e = launch_u32_fill(counts, 0u, nbk + 1, st)) != hipSuccess ||
         (e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, counts, p, err, st)) !=
-            hipSuccess)
+        (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, counts, p, err,
+                                prep_blocks, st)) != hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
     e = hipcub::DeviceScan::ExclusiveSum(scan_tmp_.ptr, tmp_bytes, counts, offsets, (int)(nbk + 1), st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket scan", e);
     e = hipMemcpyAsync(counts, offsets, (nbk + 1) * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return be_->fail(GpuError::Memory, "MSM cursor copy", e);
     if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint32_t*)entries_.ptr, (uint32_t*)keys_.ptr, p,
-                                st)) != hipSuccess)
+                                sort_blocks, st)) != hipSuccess)
         return be_->fail(GpuError::Launch, "MSM scatter", e);
     // bucket sums: empty buckets stay at infinity; level 0 over the sorted
     // entries (total = offsets[nbk]), then chunked reductions of the compacted

@@ -41,9 +41,10 @@ MsmPlan msm_plan(size_t n, int c);
 
 // Device launchers (msm.hip)
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* counts, const MsmPlan& p, uint32_t* err, hipStream_t st);
+                              uint32_t* counts, const MsmPlan& p, uint32_t* err, unsigned max_blocks,
+                              hipStream_t st);
 hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
-                              const MsmPlan& p, hipStream_t st);
+                              const MsmPlan& p, unsigned max_blocks, hipStream_t st);
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
                              hipStream_t st);

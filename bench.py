@@ -74,6 +74,24 @@ def bench_serve(seconds: float, ecdsa_items: int, schnorr_items: int) -> dict:
     return out
 
 
+def tuning_in_effect() -> dict:
+    """Launch parameters the library read at context creation (the
+    autotuner's file beside the library, or UFSECP_TUNING_FILE)."""
+    p = Path(os.environ.get("UFSECP_TUNING_FILE") or ROOT / "ultrafastsecp256k1_amd" / "ufsecp_tuning.txt")
+    out = {"file": str(p.relative_to(ROOT)) if p.is_relative_to(ROOT) else str(p), "values": {}}
+    if p.is_file():
+        msm = {}
+        for line in p.read_text().splitlines():
+            f = line.split()
+            if len(f) >= 2 and not f[0].startswith("#"):
+                if f[0] == "msm_window" and len(f) == 3:
+                    msm[f[1]] = int(f[2])
+                else:
+                    out["values"][f[0]] = int(f[1])
+        out["values"]["msm_window_by_log2n"] = msm
+    return out
+
+
 def measure_valu_peak() -> dict:
     """Run the v_mad_u64_u32 throughput probe; returns its JSON records."""
     exe = ROOT / "ultrafastsecp256k1_amd" / "valu_peak"
@@ -316,6 +334,7 @@ def main() -> int:
                    "batch": n, "kind": kind},
         "parity": parity, "roofline": roofline, "cpu_baseline": cpu, "latency": latency, "legs": legs,
         "ctx_create_ms": round(ctx_ms, 1),
+        "tuning": tuning_in_effect(),
         "valu_peak": {k: v for k, v in peak.items() if k != "device"},
         "device": peak["device"],
     }

@@ -47,3 +47,28 @@ def test_msm_infinity_is_arith_error(gpu):
     sc2[63] = np.frombuffer(s63.to_bytes(32, "big"), np.uint8)
     rc, out = gpu.msm_rc(sc2, pts)
     assert rc == UFSECP_ERR_ARITH and out == b"\x00" * 33
+
+
+@pytest.mark.parametrize("c", [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
+def test_msm_every_window_width(gpu, c):
+    """Explicit window widths: every c gives the closed-form point, including
+    the top window's LDS counting (257 - c(W-1) <= 8 bits) and the small
+    bucket counts of c < 4."""
+    n = 777
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + c)
+    e = wl.msm_closed_form_scalar(sc, ks)
+    rc, out = gpu.msm_rc(sc, pts, n, c)
+    assert rc == UFSECP_OK
+    assert out == compress(mul(e))
+
+
+@pytest.mark.parametrize("c", [10, 12, 14, 16])
+def test_msm_window_widths_at_2p18(gpu, c):
+    """The configuration sweep's widths on a size where the top window's
+    buckets hold hundreds of thousands of terms."""
+    n = 1 << 18
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 100 + c)
+    e = wl.msm_closed_form_scalar(sc, ks)
+    rc, out = gpu.msm_rc(sc, pts, n, c)
+    assert rc == UFSECP_OK
+    assert out == compress(mul(e))

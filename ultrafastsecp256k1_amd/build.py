@@ -34,7 +34,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
-GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "msm.hip", "backend.cpp", "server.cpp", "ufsecp_gpu_abi.cpp"]
+GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp",
+               "ufsecp_gpu_abi.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -118,8 +118,10 @@ GpuError RocmBackend::init(uint32_t device_index) {
     e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(GpuError::Queue, "hipStreamCreate", e);
 
-    // Fixed-base comb table for G: window width from UFSECP_GCOMB_W (4..16).
-    const int w = env_int("UFSECP_GCOMB_W", 16, 4, 16);
+    // Fixed-base comb table for G: width from the tuning file, overridden by
+    // UFSECP_GCOMB_W (4..16).
+    tuning_ = load_tuning();
+    const int w = env_int("UFSECP_GCOMB_W", tuning_.gcomb_w, 4, 16);
     const size_t entries = gcomb_entries(w);
     e = hipMalloc(&d_gcomb_, entries * 64);
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);
@@ -138,7 +140,9 @@ GpuError RocmBackend::init(uint32_t device_index) {
     cus_ = p.multiProcessorCount;
     unsigned max_grid = 0;
     for (int k = 0; k < 3; ++k) {
-        verify_grid_[k] = (unsigned)(cus_ * verify_blocks_per_cu(k));
+        int bpc = verify_blocks_per_cu(k);
+        if (tuning_.verify_blocks_per_cu > 0 && tuning_.verify_blocks_per_cu < bpc) bpc = tuning_.verify_blocks_per_cu;
+        verify_grid_[k] = (unsigned)(cus_ * bpc);
         if (verify_grid_[k] > max_grid) max_grid = verify_grid_[k];
     }
     if ((e = d_slots_.ensure((size_t)max_grid * kVerifyBlock * kSlotBytes)) != hipSuccess)

@@ -16,6 +16,7 @@
 #include <string>
 
 #include "kernels.h"
+#include "tuning.hpp"
 
 namespace ufs {
 
@@ -97,6 +98,7 @@ public:
     const GcombDesc& gcomb() const { return gdesc_; }
     const uint32_t* bip340_midstate() const { return d_mid_; }
     int device() const { return device_; }
+    const Tuning& tuning() const { return tuning_; }
     int compute_units() const { return cus_; }
     unsigned verify_grid(int kind) const { return verify_grid_[kind]; }
 
@@ -133,6 +135,7 @@ private:
     hipStream_t graph_stream_ = nullptr;
     const void* graph_descs_ = nullptr;
     const void* graph_tiles_ = nullptr;
+    Tuning tuning_;
     unsigned verify_grid_[3] = {0, 0, 0};   // blocks: ECDSA, BIP-340, mixed
     DeviceBuffer d_slots_;                   // kSlotBytes per lane of the largest grid
     hipEvent_t verify_done_ = nullptr;

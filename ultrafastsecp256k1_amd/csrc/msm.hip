@@ -463,7 +463,12 @@ int msm_default_window(size_t n) {
     return c;
 }
 
-MsmPlan msm_plan(size_t n, int c) {
+MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
+    if (c <= 0 && tun) {
+        int lg = 0;
+        while (((size_t)1 << (lg + 1)) <= n) ++lg;
+        if (lg < 40) c = tun->msm_window[lg];
+    }
     if (c <= 0) c = msm_default_window(n);
     MsmPlan p;
     p.c = c;
@@ -483,7 +488,7 @@ MsmEngine::~MsmEngine() {
 GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                             int32_t* d_status, int c, hipStream_t st) {
     if (c != 0 && (c < 2 || c > 20)) return be_->fail(GpuError::BadInput, "MSM window width out of range");
-    const MsmPlan p = msm_plan(n, c);
+    const MsmPlan p = msm_plan(n, c, &be_->tuning());
     const size_t nbk = (size_t)p.windows * p.buckets;
     const size_t nseg = (size_t)p.windows * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac);

@@ -37,7 +37,8 @@ struct MsmPlan {
 };
 
 int msm_default_window(size_t n);
-MsmPlan msm_plan(size_t n, int c);
+// c <= 0: the tuned window for n's size band, else the size formula.
+MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 
 // Device launchers (msm.hip)
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,

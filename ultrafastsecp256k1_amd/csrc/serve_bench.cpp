@@ -13,7 +13,7 @@
 //
 //   serve_bench [--seconds S] [--loads 0.5,0.8,0.95] [--ecdsa-items N]
 //               [--schnorr-items N] [--max-size N] [--cache-log2 K]
-//               [--groups G] [--budget-mb M] [--no-plain]
+//               [--groups G] [--budget-mb M] [--no-plain] [--peak-only]
 // Prints one JSON object.
 #include <algorithm>
 #include <atomic>
@@ -78,9 +78,10 @@ struct Args {
     size_t ecdsa_items = 4u << 20, schnorr_items = 2u << 20;
     uint32_t max_size = 8192;
     uint32_t cache_log2 = 20;
-    uint32_t groups = 2;
+    uint32_t groups = 0;
     uint64_t budget_mb = 512;
     bool plain = true;
+    bool peak_only = false;
 };
 
 [[noreturn]] void die(const char* what, int rc) {
@@ -345,6 +346,11 @@ int main(int argc, char** argv) {
         else if (s == "--groups") a.groups = (uint32_t)std::atoi(val());
         else if (s == "--budget-mb") a.budget_mb = std::strtoull(val(), nullptr, 10);
         else if (s == "--no-plain") a.plain = false;
+        else if (s == "--peak-only") {
+            a.peak_only = true;
+            a.plain = false;
+            a.loads.clear();
+        }
         else if (s == "--loads") {
             a.loads.clear();
             std::string v = val();
@@ -391,7 +397,7 @@ int main(int argc, char** argv) {
 
     ufsecp_gpu_server_config_t cfg{};
     cfg.hbm_budget_bytes = a.budget_mb << 20;
-    cfg.inflight_groups = a.groups;
+    cfg.inflight_groups = a.groups;   // 0: the tuning file's value
     cfg.key_cache_log2 = a.cache_log2;
     ufsecp_gpu_server* srv = nullptr;
     if ((rc = ufsecp_gpu_ext_server_create(ctx, &cfg, &srv)) != UFSECP_OK) die("server_create", rc);

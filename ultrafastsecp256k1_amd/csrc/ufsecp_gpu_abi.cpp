@@ -456,6 +456,7 @@ ufsecp_error_t ufsecp_gpu_ext_server_create(ufsecp_gpu_ctx* ctx, const ufsecp_gp
     try {
         if (!ctx->backend->is_ready()) return UFSECP_ERR_GPU_DEVICE;
         ufs::ServerConfig c;
+        c.groups = (unsigned)ctx->backend->tuning().server_groups;
         if (cfg) {
             if (cfg->hbm_budget_bytes) c.hbm_budget = (size_t)cfg->hbm_budget_bytes;
             if (cfg->inflight_groups) c.groups = cfg->inflight_groups;

@@ -232,7 +232,7 @@ def main() -> int:
     tuned: dict = {}
     n = 1 << (17 if args.quick else 20)
     t0 = time.perf_counter()
-    tune_comb(tuned, n, (8, 10, 12, 14, 16))
+    tune_comb(tuned, n, (12, 14, 16, 17, 18))
     tune_grid(tuned, n, 500 if args.quick else 2000)
     tune_msm(tuned, (8, 10, 12, 14, 16) if args.quick else (8, 10, 12, 14, 16, 18, 20))
     tune_server(tuned, *((1 << 19, 1 << 18) if args.quick else (1 << 21, 1 << 20)))

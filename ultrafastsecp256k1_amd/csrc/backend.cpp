@@ -119,9 +119,9 @@ GpuError RocmBackend::init(uint32_t device_index) {
     if (e != hipSuccess) return fail(GpuError::Queue, "hipStreamCreate", e);
 
     // Fixed-base comb table for G: width from the tuning file, overridden by
-    // UFSECP_GCOMB_W (4..16).
+    // UFSECP_GCOMB_W (4..20).
     tuning_ = load_tuning();
-    const int w = env_int("UFSECP_GCOMB_W", tuning_.gcomb_w, 4, 16);
+    const int w = env_int("UFSECP_GCOMB_W", tuning_.gcomb_w, 4, 20);
     const size_t entries = gcomb_entries(w);
     e = hipMalloc(&d_gcomb_, entries * 64);
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);

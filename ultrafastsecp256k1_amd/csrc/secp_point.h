@@ -226,13 +226,18 @@ UFS_DEV void jac_madd_mem_body(jac& out, const jac& a, const aff* bp, u32 flags,
     fe_sub(t, t, z1z1);
     fe_sub(out.z, t, hh);      // Z3 = (Z1 + H)^2 - Z1Z1 - HH
     out.inf = 0;
-    if (a.inf) jac_from_aff(out, b);
-    need_dbl = degen & same;
-    if (degen) {
-        if (same) {
-            out = a;
-        } else {
-            jac_set_inf(out);
+    need_dbl = false;
+    // Rare cases (a = inf, a = +-b) behind a wave-uniform test, so the common
+    // path carries no per-lane selects.
+    if (__any(a.inf | degen)) {
+        if (a.inf) jac_from_aff(out, b);
+        need_dbl = degen & same;
+        if (degen) {
+            if (same) {
+                out = a;
+            } else {
+                jac_set_inf(out);
+            }
         }
     }
 }

@@ -33,7 +33,7 @@ void load(Tuning& t) {
         int a = 0, b = 0;
         const int got = std::sscanf(line, "%63s %d %d", key, &a, &b);
         if (got < 2 || key[0] == '#') continue;
-        if (!std::strcmp(key, "gcomb_w") && a >= 4 && a <= 16) t.gcomb_w = a;
+        if (!std::strcmp(key, "gcomb_w") && a >= 4 && a <= 20) t.gcomb_w = a;
         else if (!std::strcmp(key, "verify_blocks_per_cu") && a >= 0 && a <= 8) t.verify_blocks_per_cu = a;
         else if (!std::strcmp(key, "server_groups") && a >= 1 && a <= 4) t.server_groups = a;
         else if (!std::strcmp(key, "msm_window") && got == 3 && a >= 0 && a < 40 && b >= 2 && b <= 20)

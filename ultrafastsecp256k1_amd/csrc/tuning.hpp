@@ -1,7 +1,7 @@
 // Launch parameters chosen by the project's autotuner
 // (ultrafastsecp256k1_amd/autotune.py), read at every context creation from a
 // small text file of "key value" lines:
-//   gcomb_w <4..16>                 width of the fixed-base G comb table
+//   gcomb_w <4..20>                 width of the fixed-base G comb table
 //   verify_blocks_per_cu <1..8>     persistent verify grid (0 = occupancy)
 //   msm_window <log2 n> <c>         Pippenger window for n in [2^k, 2^(k+1))
 //   server_groups <1..4>            launch groups in flight in the server

@@ -112,6 +112,36 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_schnorr_verify_batch(ufsecp_gpu_ctx* ctx, c
                                                           const uint8_t* pubkeys_x32, const uint8_t* sigs64,
                                                           size_t count, uint8_t* out_results);
 
+/* libbitcoin opaque-signature ECDSA rows: item i is the `stride`-byte row at
+ * rows + i*stride holding 32-byte digest | 33-byte compressed pubkey |
+ * 64-byte opaque signature (r, s as 256-bit little-endian integers) | tail.
+ * stride < 129 or count > 2^26 -> BAD_INPUT.  Verdicts equal the reference
+ * CPU ufsecp_ecdsa_verify_opaque_rows (impl/ufsecp_ecdsa.cpp:412-452): r, s
+ * strict in [1,n), high-S accepted, strict pubkey.  Replaces
+ * ufsecp_gpu_ecdsa_verify_opaque_rows (ufsecp_gpu.h:228-233). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdsa_verify_opaque_rows(ufsecp_gpu_ctx* ctx, const uint8_t* rows,
+                                                              size_t stride, size_t count, uint8_t* out_results);
+
+/* Alias of the above.  Replaces ufsecp_gpu_ecdsa_verify_lbtc_rows
+ * (ufsecp_gpu.h:236-241). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdsa_verify_lbtc_rows(ufsecp_gpu_ctx* ctx, const uint8_t* rows, size_t stride,
+                                                            size_t count, uint8_t* out_results);
+
+/* The same opaque ECDSA verification over three packed columns: digests32
+ * N*32, pubkeys33 N*33, sigs64 N*64 opaque.  Verdicts equal the reference CPU
+ * ufsecp_ecdsa_verify_opaque_batch (impl/ufsecp_ecdsa.cpp:368-410).  Replaces
+ * ufsecp_gpu_ecdsa_verify_lbtc_columns (ufsecp_gpu.h:278-284). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdsa_verify_lbtc_columns(ufsecp_gpu_ctx* ctx, const uint8_t* digests32,
+                                                               const uint8_t* pubkeys33, const uint8_t* sigs64,
+                                                               size_t count, uint8_t* out_results);
+
+/* BIP-340 over columns digests32 | xonly32 | sigs64: the verdicts of
+ * ufsecp_gpu_schnorr_verify_batch (strict s in [1,n), r < p).  Replaces
+ * ufsecp_gpu_schnorr_verify_lbtc_columns (ufsecp_gpu.h:299-305). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_schnorr_verify_lbtc_columns(ufsecp_gpu_ctx* ctx, const uint8_t* digests32,
+                                                                 const uint8_t* xonly32, const uint8_t* sigs64,
+                                                                 size_t count, uint8_t* out_results);
+
 /* In-place verdict collapse: key_buffer bytes are caller-seeded non-zero
  * markers; a VALID item zeroes its byte, an invalid one leaves it.  Not
  * pre-cleared.  Replaces ufsecp_gpu_ecdsa_verify_collect (ufsecp_gpu.h:326-332). */

@@ -31,10 +31,19 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_schnorr_verify_device(ufsecp_gpu_ctx* c
                                                                size_t count, uint8_t* d_out_results, void* stream);
 
 /* One batch of a multi-batch verification (all pointers are device
- * addresses; strides as in the single-batch calls: msgs 32 B, pubkeys 33 B for
- * ECDSA or 32 B for BIP-340, sigs 64 B, out 1 B per item). */
-#define UFSECP_GPU_KIND_ECDSA   0u
-#define UFSECP_GPU_KIND_SCHNORR 1u
+ * addresses).  Kinds:
+ *   UFSECP_GPU_KIND_ECDSA         as ufsecp_gpu_ecdsa_verify_batch
+ *   UFSECP_GPU_KIND_SCHNORR       as ufsecp_gpu_schnorr_verify_batch
+ *   UFSECP_GPU_KIND_ECDSA_OPAQUE  as ufsecp_gpu_ecdsa_verify_lbtc_columns
+ *                                 (opaque little-endian r, s; high-S accepted)
+ * With row_stride 0 the inputs are packed columns: msgs 32 B, pubkeys 33 B
+ * (ECDSA kinds) or 32 B (BIP-340), sigs 64 B, out 1 B per item.  A non-zero
+ * row_stride (>= 129, opaque kind only) is the byte distance between items of
+ * all three inputs, as for interleaved rows (msg_hashes32 = rows,
+ * pubkeys = rows + 32, sigs64 = rows + 65).  `reserved` must be zero. */
+#define UFSECP_GPU_KIND_ECDSA        0u
+#define UFSECP_GPU_KIND_SCHNORR      1u
+#define UFSECP_GPU_KIND_ECDSA_OPAQUE 2u
 typedef struct {
     uint32_t kind;
     uint32_t count;
@@ -42,6 +51,8 @@ typedef struct {
     const uint8_t* pubkeys;
     const uint8_t* sigs64;
     uint8_t* out_results;
+    uint32_t row_stride;
+    uint32_t reserved;
 } ufsecp_gpu_verify_batch_t;
 
 /* Verify `nbatches` batches of uneven sizes and mixed kinds in ONE persistent

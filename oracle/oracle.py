@@ -37,6 +37,7 @@ def lib() -> ctypes.CDLL:
         L.orc_ecdsa_sign.argtypes = [vp, vp, vp, vp]
         L.orc_schnorr_sign.argtypes = [vp, vp, vp, vp, vp]
         L.orc_verify_batch.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp, ctypes.c_int]
+        L.orc_ecdsa_verify_opaque.argtypes = [vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -51,6 +52,16 @@ def _p(b):
 
 def ecdsa_verify(msg: bytes, sig: bytes, pub33: bytes) -> int:
     return lib().orc_ecdsa_verify(msg, sig, pub33)
+
+
+def ecdsa_verify_opaque(msg: bytes, opaque_sig: bytes, pub33: bytes) -> int:
+    """Opaque-signature ECDSA (little-endian r, s; high-S accepted)."""
+    return lib().orc_ecdsa_verify_opaque(msg, opaque_sig, pub33)
+
+
+def compact_to_opaque(sig64: bytes) -> bytes:
+    """Compact big-endian r||s -> opaque form (each scalar little-endian)."""
+    return bytes(sig64[:32])[::-1] + bytes(sig64[32:64])[::-1]
 
 
 def schnorr_verify(msg: bytes, sig: bytes, px: bytes) -> int:
@@ -88,8 +99,15 @@ def verify_batch(kind: str, msgs: np.ndarray, pubs: np.ndarray, sigs: np.ndarray
     out = np.zeros(n, np.uint8)
     msgs, pubs, sigs = (np.ascontiguousarray(a) for a in (msgs, pubs, sigs))
     threads = threads or min(16, os.cpu_count() or 1)
-    lib().orc_verify_batch(1 if kind == "schnorr" else 0, _p(msgs), _p(pubs), _p(sigs), n, _p(out), threads)
+    code = {"ecdsa": 0, "schnorr": 1, "ecdsa_opaque": 2}[kind]
+    lib().orc_verify_batch(code, _p(msgs), _p(pubs), _p(sigs), n, _p(out), threads)
     return out
+
+
+def verify_rows(rows: np.ndarray) -> np.ndarray:
+    """Opaque ECDSA over (count, stride) rows: 32 digest | 33 pubkey | 64 sig."""
+    rows = np.ascontiguousarray(rows)
+    return verify_batch("ecdsa_opaque", rows[:, :32], rows[:, 32:65], rows[:, 65:129])
 
 
 REF_LIB_PATH = HERE / "_ref" / "libufsecp_ref.so"
@@ -123,3 +141,39 @@ def ref_verify_batch(kind: str, msgs: np.ndarray, pubs: np.ndarray, sigs: np.nda
 def verify_range(kind: str, msgs, pubs, sigs) -> int:
     """Single-threaded verification of a slice; returns the number valid."""
     return int(verify_batch(kind, msgs, pubs, sigs, threads=1).sum())
+
+
+def ref_verify_opaque(rows=None, stride=0, msgs=None, pubs=None, sigs=None, count=None):
+    """Opaque ECDSA through the reference CPU library (oracle/_ref):
+    ufsecp_ecdsa_verify_opaque_rows for `rows`, else
+    ufsecp_ecdsa_verify_opaque_batch over the three columns.  Returns
+    (ABI code, verdicts)."""
+    L = lib()
+    fn = L.orc_ref_verify_opaque
+    vp = ctypes.c_void_p
+    fn.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, vp, vp, vp, ctypes.c_size_t, vp]
+    if rows is not None:
+        rows = np.ascontiguousarray(rows)
+        n = rows.shape[0] if count is None else count
+        out = np.zeros(max(n, 1), np.uint8)
+        rc = fn(str(REF_LIB_PATH).encode(), _p(rows), stride, None, None, None, n, _p(out))
+    else:
+        msgs, pubs, sigs = (np.ascontiguousarray(a) for a in (msgs, pubs, sigs))
+        n = msgs.shape[0] if count is None else count
+        out = np.zeros(max(n, 1), np.uint8)
+        rc = fn(str(REF_LIB_PATH).encode(), None, 0, _p(msgs), _p(pubs), _p(sigs), n, _p(out))
+    if rc < 0:
+        raise RuntimeError(f"reference driver failed ({rc})")
+    return rc, out[:n]
+
+
+def ref_compact_to_opaque(sig64: bytes):
+    """The reference's ufsecp_ecdsa_sig_compact_to_opaque: (code, 64 bytes)."""
+    L = lib()
+    fn = L.orc_ref_compact_to_opaque
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]
+    out = ctypes.create_string_buffer(64)
+    rc = fn(str(REF_LIB_PATH).encode(), _p(sig64), out)
+    if rc < 0:
+        raise RuntimeError(f"reference driver failed ({rc})")
+    return rc, out.raw

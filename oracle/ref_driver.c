@@ -6,6 +6,7 @@
  * reference requires (include/ufsecp/ufsecp.h: contexts are single-thread). */
 #include <dlfcn.h>
 #include <pthread.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <time.h>
@@ -78,4 +79,47 @@ __attribute__((visibility("default"))) double orc_ref_verify_batch(const char* l
     clock_gettime(CLOCK_MONOTONIC, &t1);
     if (rc) return -3.0;
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+typedef int (*rows_fn)(void*, const uint8_t*, size_t, size_t, uint8_t*);
+typedef int (*cols_fn)(void*, const uint8_t*, const uint8_t*, const uint8_t*, size_t, uint8_t*);
+typedef int (*conv_fn)(void*, const uint8_t*, uint8_t*);
+
+/* Opaque-signature ECDSA through the reference CPU C ABI, single-threaded:
+ * rows != NULL -> ufsecp_ecdsa_verify_opaque_rows(rows, stride, count),
+ * else ufsecp_ecdsa_verify_opaque_batch(msgs, pubs, sigs, count).  Returns the
+ * ABI code, or a negative value if the library or symbol is missing. */
+__attribute__((visibility("default"))) int orc_ref_verify_opaque(const char* lib_path, const uint8_t* rows,
+                                                                 size_t stride, const uint8_t* msgs,
+                                                                 const uint8_t* pubs, const uint8_t* sigs,
+                                                                 size_t count, uint8_t* out) {
+    void* h = dlopen(lib_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    ctx_create_fn create = (ctx_create_fn)dlsym(h, "ufsecp_ctx_create");
+    ctx_destroy_fn destroy = (ctx_destroy_fn)dlsym(h, "ufsecp_ctx_destroy");
+    rows_fn vrows = (rows_fn)dlsym(h, "ufsecp_ecdsa_verify_opaque_rows");
+    cols_fn vcols = (cols_fn)dlsym(h, "ufsecp_ecdsa_verify_opaque_batch");
+    if (!create || !vrows || !vcols) return -2;
+    void* ctx = 0;
+    if (create(&ctx) != 0) return -3;
+    const int rc = rows ? vrows(ctx, rows, stride, count, out) : vcols(ctx, msgs, pubs, sigs, count, out);
+    if (destroy) destroy(ctx);
+    return rc;
+}
+
+/* The reference's compact (big-endian r||s) -> opaque conversion,
+ * ufsecp_ecdsa_sig_compact_to_opaque.  Returns the ABI code (< 0: missing). */
+__attribute__((visibility("default"))) int orc_ref_compact_to_opaque(const char* lib_path, const uint8_t* sig64,
+                                                                     uint8_t* opaque64) {
+    void* h = dlopen(lib_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    ctx_create_fn create = (ctx_create_fn)dlsym(h, "ufsecp_ctx_create");
+    ctx_destroy_fn destroy = (ctx_destroy_fn)dlsym(h, "ufsecp_ctx_destroy");
+    conv_fn conv = (conv_fn)dlsym(h, "ufsecp_ecdsa_sig_compact_to_opaque");
+    if (!create || !conv) return -2;
+    void* ctx = 0;
+    if (create(&ctx) != 0) return -3;
+    const int rc = conv(ctx, sig64, opaque64);
+    if (destroy) destroy(ctx);
+    return rc;
 }

@@ -9,6 +9,10 @@
  *                      low-S), ecdsa.cpp:891-927 (strict compressed key),
  *                      ecdsa.cpp:929-952 (u1 = z/s, u2 = r/s), ecdsa.cpp:764-850
  *                      (x(R) == r including the r + n case)
+ *   orc_ecdsa_verify_opaque  src/cpu/src/impl/ufsecp_ecdsa.cpp:105-117,
+ *                      128-140, 180-189 (opaque little-endian r,s strict in
+ *                      [1,n), s normalised: high-S accepted), :412-452
+ *                      (per-row verdict); then the orc_ecdsa_verify core
  *   orc_schnorr_verify src/cpu/src/impl/ufsecp_ecdsa.cpp:992-1019,
  *                      schnorr.cpp:355-371 (r < p, 0 < s < n), :275-300
  *                      (lift_x, x < p), :745-806 (R = sG - eP, even y, x == r)
@@ -330,10 +334,9 @@ static void hmac(u8 out[32], const u8 key[32], const u8* m, size_t nm) {
 }
 
 /* ---- verification ------------------------------------------------------- */
-ORC_API int orc_ecdsa_verify(const u8* msg32, const u8* sig64, const u8* pub33) {
-    u256 r = be_load(sig64), s = be_load(sig64 + 32);
+static int ecdsa_core(const u8* msg32, u256 r, u256 s, const u8* pub33, int require_low_s) {
     if (u_zero(r) || u_cmp(r, N_) >= 0 || u_zero(s) || u_cmp(s, N_) >= 0) return 0;
-    if (u_cmp(s, NH_) > 0) return 0;                       /* low-S required */
+    if (require_low_s && u_cmp(s, NH_) > 0) return 0;
     aff Q;
     if (!pub33_parse(&Q, pub33)) return 0;
     u256 z = s_from_bytes(msg32), w = s_inv(s);
@@ -345,6 +348,27 @@ ORC_API int orc_ecdsa_verify(const u8* msg32, const u8* sig64, const u8* pub33) 
     u256 rn;
     if (!u_add(&rn, r, N_) && u_cmp(rn, P_) < 0 && u_cmp(f_mul(rn, z2), R.x) == 0) return 1;
     return 0;
+}
+
+ORC_API int orc_ecdsa_verify(const u8* msg32, const u8* sig64, const u8* pub33) {
+    return ecdsa_core(msg32, be_load(sig64), be_load(sig64 + 32), pub33, 1);
+}
+
+/* 256-bit little-endian integer (the opaque signature scalar layout). */
+static u256 le_load(const u8* b) {
+    u256 r;
+    for (int i = 0; i < 4; ++i) {
+        u64 v = 0;
+        for (int j = 7; j >= 0; --j) v = (v << 8) | b[8 * i + j];
+        r.v[i] = v;
+    }
+    return r;
+}
+
+/* Opaque-signature ECDSA: r, s little-endian, strict in [1,n); high-S is
+ * accepted (the reference normalises s, which keeps x(R)). */
+ORC_API int orc_ecdsa_verify_opaque(const u8* msg32, const u8* opaque64, const u8* pub33) {
+    return ecdsa_core(msg32, le_load(opaque64), le_load(opaque64 + 32), pub33, 0);
 }
 
 ORC_API int orc_schnorr_verify(const u8* msg32, const u8* sig64, const u8* px32) {
@@ -458,7 +482,7 @@ ORC_API int orc_schnorr_sign(const u8* msg32, const u8* d32, const u8* aux32, u8
 
 /* ---- multi-threaded batch verify (CPU baseline / large parity checks) ---- */
 typedef struct {
-    int kind;   /* 0 ECDSA, 1 Schnorr */
+    int kind;   /* 0 ECDSA, 1 Schnorr, 2 opaque ECDSA */
     const u8 *msgs, *pubs, *sigs;
     u8* out;
     size_t lo, hi;
@@ -466,10 +490,13 @@ typedef struct {
 
 static void* run_job(void* arg) {
     job_t* j = (job_t*)arg;
-    const size_t ps = j->kind ? 32 : 33;
-    for (size_t i = j->lo; i < j->hi; ++i)
-        j->out[i] = (u8)(j->kind ? orc_schnorr_verify(j->msgs + 32 * i, j->sigs + 64 * i, j->pubs + ps * i)
-                                 : orc_ecdsa_verify(j->msgs + 32 * i, j->sigs + 64 * i, j->pubs + ps * i));
+    const size_t ps = j->kind == 1 ? 32 : 33;
+    for (size_t i = j->lo; i < j->hi; ++i) {
+        const u8 *m = j->msgs + 32 * i, *sg = j->sigs + 64 * i, *pk = j->pubs + ps * i;
+        j->out[i] = (u8)(j->kind == 1   ? orc_schnorr_verify(m, sg, pk)
+                         : j->kind == 2 ? orc_ecdsa_verify_opaque(m, sg, pk)
+                                        : orc_ecdsa_verify(m, sg, pk));
+    }
     return 0;
 }
 

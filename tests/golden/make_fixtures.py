@@ -19,8 +19,14 @@ Runs only in the container that holds the reference sources: it loads
   (the 70*G case of audit/test_gpu_ops_equivalence.cpp:353-397, random small
   MSMs, zero scalars, repeats, cancellation to infinity, invalid inputs) with
   the reference's verdicts / results / error codes.
+* ``opaque_vectors.json`` -- libbitcoin opaque-signature ECDSA items (r, s as
+  little-endian 256-bit integers): valid, high-S, r/s = 0, = n, >= n, a valid
+  small-s signature and its s + n twin, endianness swaps, bad keys, with the
+  verdicts of the reference CPU ufsecp_ecdsa_verify_opaque_rows (rows of
+  stride 129 and 137) and ufsecp_ecdsa_verify_opaque_batch (columns), plus
+  ufsecp_ecdsa_sig_compact_to_opaque outputs that fix the byte layout.
 
-Usage: python tests/golden/make_fixtures.py
+Usage: python tests/golden/make_fixtures.py [fixture name ...]
 """
 from __future__ import annotations
 
@@ -89,6 +95,22 @@ class Ref:
 
     def schnorr_verify(self, msg: bytes, sig: bytes, px: bytes) -> int:
         return 1 if self.lib.ufsecp_schnorr_verify(self.ctx(), msg, sig, px) == 0 else 0
+
+    def compact_to_opaque(self, sig: bytes):
+        out = ctypes.create_string_buffer(64)
+        rc = self.lib.ufsecp_ecdsa_sig_compact_to_opaque(self.ctx(), sig, out)
+        return rc, out.raw
+
+    def verify_opaque_rows(self, rows: bytes, stride: int, n: int):
+        out = ctypes.create_string_buffer(n)
+        rc = self.lib.ufsecp_ecdsa_verify_opaque_rows(self.ctx(), rows, ctypes.c_size_t(stride),
+                                                      ctypes.c_size_t(n), out)
+        return rc, list(out.raw)
+
+    def verify_opaque_batch(self, msgs: bytes, pubs: bytes, sigs: bytes, n: int):
+        out = ctypes.create_string_buffer(n)
+        rc = self.lib.ufsecp_ecdsa_verify_opaque_batch(self.ctx(), msgs, pubs, sigs, ctypes.c_size_t(n), out)
+        return rc, list(out.raw)
 
     def msm(self, scalars: bytes, points: bytes, n: int):
         out = ctypes.create_string_buffer(33)
@@ -335,13 +357,100 @@ def edge_cases(ref: Ref) -> dict:
     return {"ecdsa": ecdsa, "schnorr": schnorr, "msm": msm}
 
 
+def opaque_vectors(ref: Ref) -> dict:
+    n, p = N_ORDER, P_FIELD
+    b32 = lambda v: (v % (1 << 256)).to_bytes(32, "big")  # noqa: E731
+    le = lambda v: v.to_bytes(32, "little")  # noqa: E731  (v < 2^256)
+    d = b32(0x2468ACE13579BDF0)
+    msg = hashlib.sha256(b"opaque").digest()
+    pub = ref.pubkey33(d)
+    sig = ref.ecdsa_sign(msg, d)
+    r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+    items = []
+
+    def add(name, m, pk, opaque):
+        items.append({"name": name, "msg": m.hex(), "pub": pk.hex(), "sig": opaque.hex()})
+
+    add("valid", msg, pub, le(r) + le(s))
+    add("high_s", msg, pub, le(r) + le(n - s))
+    add("r_zero", msg, pub, le(0) + le(s))
+    add("s_zero", msg, pub, le(r) + le(0))
+    add("r_eq_n", msg, pub, le(n) + le(s))
+    add("s_eq_n", msg, pub, le(r) + le(n))
+    add("r_plus_n", msg, pub, (le(r + n) if r + n < (1 << 256) else le(n + 1)) + le(s))
+    add("r_ff", msg, pub, b"\xff" * 32 + le(s))
+    add("s_ff", msg, pub, le(r) + b"\xff" * 32)
+    add("big_endian_sig", msg, pub, sig)
+    add("r_s_swapped", msg, pub, le(s) + le(r))
+    add("prefix_04", msg, b"\x04" + pub[1:], le(r) + le(s))
+    add("prefix_flip", msg, bytes([pub[0] ^ 1]) + pub[1:], le(r) + le(s))
+    add("x_ge_p", msg, b"\x02" + b32(p + 1), le(r) + le(s))
+    x = 5
+    while pow((x ** 3 + 7) % p, (p - 1) // 2, p) == 1:
+        x += 1
+    add("x_off_curve", msg, b"\x02" + b32(x), le(r) + le(s))
+    add("wrong_msg", hashlib.sha256(b"other").digest(), pub, le(r) + le(s))
+    # A valid signature with a tiny s: pick the nonce k, then the key
+    # d = (s k - z) / r, so that s = (z + r d) / k holds.  Its s + n twin
+    # (< 2^256) names the same residue; the strict opaque parse rejects it.
+    k = 0x1F2E3D4C5B6A7988
+    kx = int.from_bytes(ref.pubkey33(b32(k))[1:], "big")
+    rk = kx % n
+    z = int.from_bytes(msg, "big") % n
+    for s_small in (5, 977, 1 << 64):
+        dk = (s_small * k - z) * pow(rk, -1, n) % n
+        pk = ref.pubkey33(b32(dk))
+        assert ref.ecdsa_verify(msg, b32(rk) + b32(s_small), pk) == 1
+        add(f"small_s_{s_small}", msg, pk, le(rk) + le(s_small))
+        add(f"small_s_{s_small}_plus_n", msg, pk, le(rk) + le(s_small + n))
+        add(f"small_s_{s_small}_high", msg, pk, le(rk) + le(n - s_small))
+    for kd in (1, 2, n - 1, (n - 1) // 2):
+        dd = b32(kd)
+        sg = ref.ecdsa_sign(msg, dd)
+        add(f"key_{kd % 1000}", msg, ref.pubkey33(dd), le(int.from_bytes(sg[:32], "big")) +
+            le(int.from_bytes(sg[32:], "big")))
+    for i in range(24):
+        m, pk, sg, corrupt = ecdsa_item(ref, wl.SEED_ECDSA + 7, i)
+        rr, ss = int.from_bytes(sg[:32], "big"), int.from_bytes(sg[32:], "big")
+        if i % 3 == 1 and 0 < ss < n:
+            ss = n - ss
+        add(f"workload_{i}_{corrupt}", m, pk, le(rr) + le(ss) if rr < (1 << 256) else sg)
+    conv = []
+    for sg in (sig, b32(r) + b32(n - s), b32(1) + b32(n - 1), b32(n - 1) + b32(1)):
+        rc, op = ref.compact_to_opaque(sg)
+        assert rc == 0, rc
+        conv.append({"compact": sg.hex(), "opaque": op.hex()})
+    cnt = len(items)
+    M = b"".join(bytes.fromhex(it["msg"]) for it in items)
+    P = b"".join(bytes.fromhex(it["pub"]) for it in items)
+    S = b"".join(bytes.fromhex(it["sig"]) for it in items)
+    rows129 = b"".join(bytes.fromhex(it["msg"] + it["pub"] + it["sig"]) for it in items)
+    rows137 = b"".join(bytes.fromhex(it["msg"] + it["pub"] + it["sig"]) + bytes([0xA5 ^ i] * 8)
+                       for i, it in enumerate(items))
+    rc1, v1 = ref.verify_opaque_rows(rows129, 129, cnt)
+    rc2, v2 = ref.verify_opaque_rows(rows137, 137, cnt)
+    rc3, v3 = ref.verify_opaque_batch(M, P, S, cnt)
+    assert rc1 == rc2 == rc3 == 0 and v1 == v2 == v3, (rc1, rc2, rc3)
+    for it, v in zip(items, v1):
+        it["verdict"] = v
+    # the all-valid fast path of the reference (one batch check)
+    ok = [it for it in items if it["verdict"] == 1]
+    rc4, v4 = ref.verify_opaque_rows(b"".join(bytes.fromhex(it["msg"] + it["pub"] + it["sig"]) for it in ok),
+                                     129, len(ok))
+    assert rc4 == 0 and all(v4), rc4
+    return {"items": items, "compact_to_opaque": conv,
+            "stride_checks": {"stride_128_rc": ref.verify_opaque_rows(rows129, 128, 1)[0]}}
+
+
 def main():
+    """Writes every fixture, or only those named on the command line
+    (e.g. ``make_fixtures.py opaque_vectors``)."""
     ref = Ref()
-    (HERE / "bip340_vectors.json").write_text(json.dumps(bip340_vectors(ref), indent=1))
-    (HERE / "rfc6979_vectors.json").write_text(json.dumps(rfc6979_vectors(ref), indent=1))
-    (HERE / "edge_cases.json").write_text(json.dumps(edge_cases(ref), indent=1))
-    (HERE / "workload_heads.json").write_text(json.dumps(workload_heads(ref), indent=1))
-    print("fixtures written to", HERE)
+    makers = {"bip340_vectors": bip340_vectors, "rfc6979_vectors": rfc6979_vectors, "edge_cases": edge_cases,
+              "workload_heads": workload_heads, "opaque_vectors": opaque_vectors}
+    for name in sys.argv[1:] or list(makers):
+        (HERE / f"{name}.json").write_text(json.dumps(makers[name](ref), indent=1))
+        print("wrote", HERE / f"{name}.json")
 
 
 if __name__ == "__main__":

@@ -33,7 +33,9 @@ def test_header_declares_reference_entry_points():
     names = declared_symbols()
     for ref_name in ("ufsecp_gpu_backend_count", "ufsecp_gpu_ctx_create", "ufsecp_gpu_ecdsa_verify_batch",
                      "ufsecp_gpu_schnorr_verify_batch", "ufsecp_gpu_msm", "ufsecp_gpu_ecdsa_verify_collect",
-                     "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_error_str"):
+                     "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_error_str",
+                     "ufsecp_gpu_ecdsa_verify_opaque_rows", "ufsecp_gpu_ecdsa_verify_lbtc_rows",
+                     "ufsecp_gpu_ecdsa_verify_lbtc_columns", "ufsecp_gpu_schnorr_verify_lbtc_columns"):
         assert ref_name in names
 
 
@@ -75,6 +77,11 @@ def test_null_context_calls():
     assert lib.ufsecp_gpu_ecdsa_verify_batch(None, buf, buf, buf, ctypes.c_size_t(4), out) == 1
     assert lib.ufsecp_gpu_schnorr_verify_batch(None, buf, buf, buf, ctypes.c_size_t(4), out) == 1
     assert lib.ufsecp_gpu_msm(None, buf, buf, ctypes.c_size_t(1), out) == 1
+    # row / column variants (reference audit/test_gpu_abi_gate.cpp:147-150)
+    assert lib.ufsecp_gpu_ecdsa_verify_opaque_rows(None, buf, ctypes.c_size_t(129), ctypes.c_size_t(1), out) == 1
+    assert lib.ufsecp_gpu_ecdsa_verify_lbtc_rows(None, buf, ctypes.c_size_t(129), ctypes.c_size_t(1), out) == 1
+    assert lib.ufsecp_gpu_ecdsa_verify_lbtc_columns(None, buf, buf, buf, ctypes.c_size_t(1), out) == 1
+    assert lib.ufsecp_gpu_schnorr_verify_lbtc_columns(None, buf, buf, buf, ctypes.c_size_t(1), out) == 1
     assert lib.ufsecp_gpu_is_ready(None) == 0
     assert lib.ufsecp_gpu_last_error(None) == 1
 

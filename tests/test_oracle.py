@@ -106,3 +106,85 @@ def test_pool_workload_items_verify(kind):
     assert np.array_equal(v, 1 - b.corrupted)
     _, counts = np.unique(b.pubkeys, axis=0, return_counts=True)
     assert counts.max() > 5
+
+
+def _opaque_columns(items):
+    m = np.frombuffer(b"".join(bytes.fromhex(c["msg"]) for c in items), np.uint8).reshape(-1, 32)
+    p = np.frombuffer(b"".join(bytes.fromhex(c["pub"]) for c in items), np.uint8).reshape(-1, 33)
+    s = np.frombuffer(b"".join(bytes.fromhex(c["sig"]) for c in items), np.uint8).reshape(-1, 64)
+    return m, p, s
+
+
+def test_opaque_vectors(golden):
+    """Opaque ECDSA verdicts of the reference CPU rows/columns ABI (high-S
+    accepted, r or s >= n rejected even when it reduces to a valid scalar)."""
+    items = golden["opaque_vectors"]["items"]
+    for c in items:
+        v = orc.ecdsa_verify_opaque(bytes.fromhex(c["msg"]), bytes.fromhex(c["sig"]), bytes.fromhex(c["pub"]))
+        assert v == c["verdict"], c["name"]
+    m, p, s = _opaque_columns(items)
+    want = np.array([c["verdict"] for c in items], np.uint8)
+    assert np.array_equal(orc.verify_batch("ecdsa_opaque", m, p, s, threads=4), want)
+    rows = np.concatenate([m, p, s, np.full((len(items), 11), 0x5A, np.uint8)], axis=1)
+    assert np.array_equal(orc.verify_rows(rows), want)
+    by = {c["name"]: c["verdict"] for c in items}
+    assert by["high_s"] == 1 and by["small_s_5_plus_n"] == 0 and by["r_plus_n"] == 0
+
+
+def test_opaque_layout(golden):
+    """compact -> opaque is the reference's ufsecp_ecdsa_sig_compact_to_opaque
+    (each 32-byte big-endian scalar stored little-endian)."""
+    for c in golden["opaque_vectors"]["compact_to_opaque"]:
+        assert orc.compact_to_opaque(bytes.fromhex(c["compact"])).hex() == c["opaque"]
+
+
+def test_opaque_against_compact_verify():
+    """Closed relation to the pinned compact verifier: an opaque signature
+    verifies iff its low-S compact form does; the high-S twin verifies too."""
+    rnd = random.Random(5)
+    for i in range(24):
+        d = rnd.randrange(1, N).to_bytes(32, "big")
+        msg = rnd.randbytes(32)
+        sig, _ = orc.ecdsa_sign(msg, d)
+        pub = orc.pubkey33(d)
+        if i % 4 == 3:
+            msg = bytes([msg[0] ^ 1]) + msg[1:]
+        r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+        want = orc.ecdsa_verify(msg, sig, pub)
+        assert orc.ecdsa_verify_opaque(msg, orc.compact_to_opaque(sig), pub) == want
+        hi = r.to_bytes(32, "little") + (N - s).to_bytes(32, "little")
+        assert orc.ecdsa_verify_opaque(msg, hi, pub) == want
+        assert orc.ecdsa_verify(msg, sig[:32] + (N - s).to_bytes(32, "big"), pub) == 0   # compact: low-S only
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="reference CPU library (oracle/_ref) not built")
+def test_opaque_rows_live_reference():
+    """Random rows at several strides through the reference CPU ABI
+    (ufsecp_ecdsa_verify_opaque_rows / _batch) against the oracle."""
+    rnd = random.Random(9)
+    n_items = 96
+    m = np.zeros((n_items, 32), np.uint8)
+    p = np.zeros((n_items, 33), np.uint8)
+    s = np.zeros((n_items, 64), np.uint8)
+    for i in range(n_items):
+        d = rnd.randrange(1, N).to_bytes(32, "big")
+        msg = rnd.randbytes(32)
+        sig, _ = orc.ecdsa_sign(msg, d)
+        op = bytearray(orc.compact_to_opaque(sig))
+        if i % 5 == 1:
+            op[32:] = (N - int.from_bytes(sig[32:], "big")).to_bytes(32, "little")
+        if i % 7 == 2:
+            op[rnd.randrange(64)] ^= 1 << rnd.randrange(8)
+        m[i] = np.frombuffer(msg, np.uint8)
+        p[i] = np.frombuffer(orc.pubkey33(d), np.uint8)
+        s[i] = np.frombuffer(bytes(op), np.uint8)
+    want = orc.verify_batch("ecdsa_opaque", m, p, s, threads=4)
+    assert 0 < want.sum() < n_items
+    for stride in (129, 130, 200):
+        rows = np.zeros((n_items, stride), np.uint8)
+        rows[:, :32], rows[:, 32:65], rows[:, 65:129] = m, p, s
+        rows[:, 129:] = 0xEE
+        rc, got = orc.ref_verify_opaque(rows=rows, stride=stride)
+        assert rc == 0 and np.array_equal(got, want), stride
+    rc, got = orc.ref_verify_opaque(msgs=m, pubs=p, sigs=s)
+    assert rc == 0 and np.array_equal(got, want)

@@ -37,7 +37,8 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_device_count", "ufsecp_gpu_device_info", "ufsecp_gpu_ctx_create",
     "ufsecp_gpu_ctx_destroy", "ufsecp_gpu_is_ready", "ufsecp_gpu_last_error",
     "ufsecp_gpu_last_error_msg", "ufsecp_gpu_error_str", "ufsecp_gpu_ecdsa_verify_batch",
-    "ufsecp_gpu_schnorr_verify_batch", "ufsecp_gpu_ecdsa_verify_collect",
+    "ufsecp_gpu_schnorr_verify_batch", "ufsecp_gpu_ecdsa_verify_opaque_rows", "ufsecp_gpu_ecdsa_verify_lbtc_rows",
+    "ufsecp_gpu_ecdsa_verify_lbtc_columns", "ufsecp_gpu_schnorr_verify_lbtc_columns", "ufsecp_gpu_ecdsa_verify_collect",
     "ufsecp_gpu_schnorr_verify_collect", "ufsecp_gpu_msm",
     "ufsecp_gpu_ext_ecdsa_verify_device", "ufsecp_gpu_ext_schnorr_verify_device",
     "ufsecp_gpu_ext_stream", "ufsecp_gpu_ext_synchronize", "ufsecp_gpu_ext_selftest",
@@ -56,11 +57,13 @@ ABI_SYMBOLS = (
 class VerifyBatchDesc(ctypes.Structure):
     """ufsecp_gpu_verify_batch_t (include/ufsecp/ufsecp_gpu_ext.h); device pointers."""
     _fields_ = [("kind", c_uint32), ("count", c_uint32), ("msg_hashes32", c_void_p),
-                ("pubkeys", c_void_p), ("sigs64", c_void_p), ("out_results", c_void_p)]
+                ("pubkeys", c_void_p), ("sigs64", c_void_p), ("out_results", c_void_p),
+                ("row_stride", c_uint32), ("reserved", c_uint32)]
 
 
 KIND_ECDSA = 0
 KIND_SCHNORR = 1
+KIND_ECDSA_OPAQUE = 2
 
 
 class ServerConfig(ctypes.Structure):
@@ -123,6 +126,10 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_error_str": (c_char_p, [c_int]),
         "ufsecp_gpu_ecdsa_verify_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ecdsa_verify_opaque_rows": (c_int, [vp, vp, c_size_t, c_size_t, vp]),
+        "ufsecp_gpu_ecdsa_verify_lbtc_rows": (c_int, [vp, vp, c_size_t, c_size_t, vp]),
+        "ufsecp_gpu_ecdsa_verify_lbtc_columns": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_schnorr_verify_lbtc_columns": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_ecdsa_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
@@ -259,6 +266,38 @@ class UfsecpGpu:
         self._check("ufsecp_gpu_schnorr_verify_batch", rc)
         return out
 
+    def ecdsa_verify_opaque_rows(self, rows, stride: int, count: Optional[int] = None, alias: bool = False):
+        """libbitcoin rows: 32 digest | 33 pubkey | 64 opaque sig | tail, `stride` bytes each
+        (ufsecp_gpu_ecdsa_verify_opaque_rows, or its ufsecp_gpu_ecdsa_verify_lbtc_rows alias)."""
+        import numpy as np
+        if count is None:
+            nbytes = getattr(rows, "nbytes", None)
+            nbytes = nbytes if nbytes is not None else len(rows)
+            count = (nbytes - 129) // stride + 1 if nbytes >= 129 and stride else 0
+        out = np.zeros(max(count, 1), dtype=np.uint8)
+        name = "ufsecp_gpu_ecdsa_verify_lbtc_rows" if alias else "ufsecp_gpu_ecdsa_verify_opaque_rows"
+        rc = getattr(self._lib, name)(self._ctx, _buf(rows), stride, count, _buf(out))
+        self._check(name, rc)
+        return out[:count]
+
+    def ecdsa_verify_lbtc_columns(self, digests32, pubkeys33, opaque_sigs64, count: Optional[int] = None):
+        import numpy as np
+        n = _nitems(digests32, 32) if count is None else count
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_ecdsa_verify_lbtc_columns(self._ctx, _buf(digests32), _buf(pubkeys33),
+                                                            _buf(opaque_sigs64), n, _buf(out))
+        self._check("ufsecp_gpu_ecdsa_verify_lbtc_columns", rc)
+        return out
+
+    def schnorr_verify_lbtc_columns(self, digests32, xonly32, sigs64, count: Optional[int] = None):
+        import numpy as np
+        n = _nitems(digests32, 32) if count is None else count
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_schnorr_verify_lbtc_columns(self._ctx, _buf(digests32), _buf(xonly32),
+                                                              _buf(sigs64), n, _buf(out))
+        self._check("ufsecp_gpu_schnorr_verify_lbtc_columns", rc)
+        return out
+
     def msm(self, scalars32, points33, n: Optional[int] = None) -> bytes:
         n = _nitems(scalars32, 32) if n is None else n
         out = ctypes.create_string_buffer(33)
@@ -300,10 +339,12 @@ class UfsecpGpu:
 
     @staticmethod
     def batch_table(batches) -> ctypes.Array:
-        """ctypes array of VerifyBatchDesc from (kind, count, d_msgs, d_pubs, d_sigs, d_out) tuples."""
+        """ctypes array of VerifyBatchDesc from (kind, count, d_msgs, d_pubs, d_sigs, d_out[, row_stride])
+        tuples."""
         arr = (VerifyBatchDesc * len(batches))()
-        for i, (kind, count, m, p, s, o) in enumerate(batches):
-            arr[i] = VerifyBatchDesc(kind, count, m, p, s, o)
+        for i, b in enumerate(batches):
+            kind, count, m, p, s, o = b[:6]
+            arr[i] = VerifyBatchDesc(kind, count, m, p, s, o, b[6] if len(b) > 6 else 0, 0)
         return arr
 
     def verify_multi_device(self, table, nbatches: int, use_graph: bool = True,

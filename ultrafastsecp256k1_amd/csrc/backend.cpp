@@ -177,7 +177,8 @@ hipError_t RocmBackend::verify_end(hipStream_t st) {
 hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStream_t st) {
     if (d.count == 0) return hipSuccess;
     const uint32_t tiles = (d.count + kTileItems - 1) / kTileItems;
-    const int kind = d.kind == 0 ? 0 : 1;
+    // kernel variant: ECDSA-only, BIP-340-only, or kind read from the batch
+    const int kind = d.kind == kKindEcdsa ? 0 : d.kind == kKindSchnorr ? 1 : 2;
     const unsigned waves_per_block = kVerifyBlock / 64;
     unsigned grid = (tiles + waves_per_block - 1) / waves_per_block;
     if (grid > verify_grid_[kind]) grid = verify_grid_[kind];
@@ -192,12 +193,12 @@ hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStrea
     return e;
 }
 
-GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
+GpuError RocmBackend::verify_host(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
                                   size_t count, uint8_t* out, bool collect) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
     hipError_t e = hipSetDevice(device_);
     if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
-    const size_t pub_stride = schnorr ? 32 : 33;
+    const size_t pub_stride = kind == kKindSchnorr ? 32 : 33;
     const size_t sz_m = count * 32, sz_p = count * pub_stride, sz_s = count * 64;
     if ((e = d_in_[0].ensure(sz_m)) != hipSuccess || (e = d_in_[1].ensure(sz_p)) != hipSuccess ||
         (e = d_in_[2].ensure(sz_s)) != hipSuccess || (e = d_out_.ensure(count)) != hipSuccess)
@@ -212,7 +213,7 @@ GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8
     const uint8_t* dp = (const uint8_t*)d_in_[1].ptr;
     const uint8_t* ds = (const uint8_t*)d_in_[2].ptr;
     uint8_t* dout = (uint8_t*)d_out_.ptr;
-    e = launch_single(BatchDesc{schnorr ? 1u : 0u, (uint32_t)count, dm, dp, ds, dout}, collect, stream_);
+    e = launch_single(BatchDesc{kind, (uint32_t)count, dm, dp, ds, dout, 0, 0}, collect, stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "verify kernel launch", e);
     e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
     if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
@@ -224,19 +225,66 @@ GpuError RocmBackend::verify_host(bool schnorr, const uint8_t* msgs, const uint8
 
 GpuError RocmBackend::ecdsa_verify_batch(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs,
                                          size_t count, uint8_t* out, bool collect) {
-    return verify_host(false, msgs, pubs33, sigs, count, out, collect);
+    return verify_host(kKindEcdsa, msgs, pubs33, sigs, count, out, collect);
 }
 
 GpuError RocmBackend::schnorr_verify_batch(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs,
                                            size_t count, uint8_t* out, bool collect) {
-    return verify_host(true, msgs, pubx, sigs, count, out, collect);
+    return verify_host(kKindSchnorr, msgs, pubx, sigs, count, out, collect);
+}
+
+GpuError RocmBackend::ecdsa_verify_opaque_columns(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs,
+                                                  size_t count, uint8_t* out) {
+    return verify_host(kKindEcdsaOpaque, msgs, pubs33, sigs, count, out, false);
+}
+
+// Row layout of the reference (gpu_backend_cuda.cu:1030-1082): rows are read
+// at `stride`, only the first 129 bytes of each row are used.  Rows up to
+// twice that long travel as one contiguous copy and are read in place at
+// their stride; longer rows are gathered by a 2-D copy into 129-byte rows, so
+// the transfer never carries much more than the used bytes.  Only
+// (count - 1) * stride + 129 bytes of the caller's buffer are read.
+GpuError RocmBackend::ecdsa_verify_opaque_rows(const uint8_t* rows, size_t stride, size_t count, uint8_t* out) {
+    constexpr size_t kRow = 32 + 33 + 64;
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    if (count == 0) {
+        clear_error();
+        return GpuError::Ok;
+    }
+    if (!rows || !out) return fail(GpuError::NullArg, "NULL buffer");
+    if (stride < kRow) return fail(GpuError::BadInput, "opaque row stride < 129");
+    if (count > SIZE_MAX / stride) return fail(GpuError::BadInput, "row byte size overflow");
+    if (count > UINT32_MAX) return fail(GpuError::BadInput, "batch count too large");
+    hipError_t e = hipSetDevice(device_);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    const bool dense = stride <= 2 * kRow;
+    const size_t pitch = dense ? stride : kRow;
+    const size_t bytes = (count - 1) * pitch + kRow;
+    if ((e = d_in_[0].ensure(bytes)) != hipSuccess || (e = d_out_.ensure(count)) != hipSuccess)
+        return fail(GpuError::Memory, "hipMalloc(row buffers)", e);
+    uint8_t* d = (uint8_t*)d_in_[0].ptr;
+    if (dense)
+        e = hipMemcpyAsync(d, rows, bytes, hipMemcpyHostToDevice, stream_);
+    else
+        e = hipMemcpy2DAsync(d, kRow, rows, stride, kRow, count, hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy rows to device", e);
+    uint8_t* dout = (uint8_t*)d_out_.ptr;
+    e = launch_single(BatchDesc{kKindEcdsaOpaque, (uint32_t)count, d, d + 32, d + 65, dout, (uint32_t)pitch, 0},
+                      false, stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "opaque-row verify launch", e);
+    e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
+    e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return fail(GpuError::Backend, "verify synchronize", e);
+    clear_error();
+    return GpuError::Ok;
 }
 
 GpuError RocmBackend::ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,
                                           size_t count, uint8_t* d_out, hipStream_t st) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
-    hipError_t e = launch_single(BatchDesc{0u, (uint32_t)count, d_msgs, d_pubs33, d_sigs, d_out}, false,
-                                 st ? st : stream_);
+    const BatchDesc d{kKindEcdsa, (uint32_t)count, d_msgs, d_pubs33, d_sigs, d_out, 0, 0};
+    hipError_t e = launch_single(d, false, st ? st : stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "ECDSA verify launch", e);
     clear_error();
     return GpuError::Ok;
@@ -245,11 +293,19 @@ GpuError RocmBackend::ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* 
 GpuError RocmBackend::schnorr_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubx, const uint8_t* d_sigs,
                                             size_t count, uint8_t* d_out, hipStream_t st) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
-    hipError_t e = launch_single(BatchDesc{1u, (uint32_t)count, d_msgs, d_pubx, d_sigs, d_out}, false,
-                                 st ? st : stream_);
+    const BatchDesc d{kKindSchnorr, (uint32_t)count, d_msgs, d_pubx, d_sigs, d_out, 0, 0};
+    hipError_t e = launch_single(d, false, st ? st : stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "Schnorr verify launch", e);
     clear_error();
     return GpuError::Ok;
+}
+
+// Why a caller's batch descriptor is unusable, or nullptr.
+static const char* batch_desc_error(const BatchDesc& d) {
+    if (d.kind > kKindEcdsaOpaque) return "batch kind must be 0 (ECDSA), 1 (BIP-340) or 2 (opaque ECDSA)";
+    if (d.stride != 0 && (d.kind != kKindEcdsaOpaque || d.stride < 129))
+        return "row stride is for opaque ECDSA batches only and must be >= 129";
+    return nullptr;
 }
 
 GpuError RocmBackend::verify_each_device(const BatchDesc* batches, size_t nbatches, hipStream_t st) {
@@ -257,7 +313,7 @@ GpuError RocmBackend::verify_each_device(const BatchDesc* batches, size_t nbatch
     st = st ? st : stream_;
     for (size_t b = 0; b < nbatches; ++b) {
         const BatchDesc& d = batches[b];
-        if (d.kind > 1) return fail(GpuError::BadInput, "batch kind must be 0 (ECDSA) or 1 (BIP-340)");
+        if (const char* why = batch_desc_error(d)) return fail(GpuError::BadInput, why);
         hipError_t e = launch_single(d, false, st);
         if (e != hipSuccess) return fail(GpuError::Launch, "per-batch verify launch", e);
     }
@@ -272,7 +328,7 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
     // tile list: 64-item tiles per batch, in batch order
     size_t ntiles = 0;
     for (size_t b = 0; b < nbatches; ++b) {
-        if (batches[b].kind > 1) return fail(GpuError::BadInput, "batch kind must be 0 (ECDSA) or 1 (BIP-340)");
+        if (const char* why = batch_desc_error(batches[b])) return fail(GpuError::BadInput, why);
         ntiles += (batches[b].count + kTileItems - 1) / kTileItems;
     }
     if (ntiles == 0) {

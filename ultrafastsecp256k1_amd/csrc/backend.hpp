@@ -80,6 +80,12 @@ public:
                                 uint8_t* out, bool collect);
     GpuError schnorr_verify_batch(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, size_t count,
                                   uint8_t* out, bool collect);
+    // libbitcoin opaque-signature ECDSA (kKindEcdsaOpaque): interleaved rows of
+    // 32-byte digest | 33-byte pubkey | 64-byte opaque sig, `stride` >= 129
+    // bytes apart, or the same three fields as packed columns.
+    GpuError ecdsa_verify_opaque_rows(const uint8_t* rows, size_t stride, size_t count, uint8_t* out);
+    GpuError ecdsa_verify_opaque_columns(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs,
+                                         size_t count, uint8_t* out);
 
     // Device-pointer variants (asynchronous on `st`, or the context stream).
     GpuError ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,
@@ -109,7 +115,7 @@ public:
     }
 
 private:
-    GpuError verify_host(bool schnorr, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
+    GpuError verify_host(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
                          size_t count, uint8_t* out, bool collect);
     // One batch through the persistent kernel (single mode).
     hipError_t launch_single(const BatchDesc& d, bool collect, hipStream_t st);

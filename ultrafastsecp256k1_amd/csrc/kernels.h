@@ -39,15 +39,29 @@ enum SelftestOp : int {
     kOpVarMul = 21,
 };
 
-// One verify batch of a multi-batch launch (device pointers; strides as in
-// the single-batch ABI: msgs 32, pubkeys 33 (ECDSA) or 32 (BIP-340), sigs 64).
+// Signature kinds of a verify batch.
+//   kKindEcdsa        compact big-endian r||s, r,s in [1,n), low-S required
+//   kKindSchnorr      BIP-340
+//   kKindEcdsaOpaque  libbitcoin opaque form: r, s as 256-bit little-endian
+//                     integers, strict [1,n), high-S accepted
+constexpr uint32_t kKindEcdsa = 0;
+constexpr uint32_t kKindSchnorr = 1;
+constexpr uint32_t kKindEcdsaOpaque = 2;
+
+// One verify batch of a multi-batch launch (device pointers).  With
+// stride == 0 the three inputs are packed columns (msgs 32 B, pubkeys 33 B for
+// ECDSA or 32 B for BIP-340, sigs 64 B per item); a non-zero stride (opaque
+// ECDSA only) is the byte distance between consecutive items of all three,
+// as in interleaved 32 | 33 | 64-byte rows.
 struct BatchDesc {
-    uint32_t kind;    // 0 = ECDSA, 1 = BIP-340
+    uint32_t kind;
     uint32_t count;
     const uint8_t* msgs;
     const uint8_t* pubs;
     const uint8_t* sigs;
     uint8_t* out;
+    uint32_t stride;
+    uint32_t reserved;
 };
 
 // 64-item tile of one batch.
@@ -83,7 +97,8 @@ struct VerifyLaunch {
     uint32_t collect;           // 1: valid -> zero the caller's byte, else keep
 };
 
-// kind: 0 = ECDSA, 1 = BIP-340, 2 = per batch (list mode).
+// Kernel variant: 0 = ECDSA only, 1 = BIP-340 only, 2 = kind read per batch
+// (list mode, and single batches of the opaque ECDSA kind).
 hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, const uint32_t* mid, unsigned grid,
                          hipStream_t st);
 int verify_blocks_per_cu(int kind);

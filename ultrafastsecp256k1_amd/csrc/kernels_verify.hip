@@ -83,22 +83,48 @@ UFS_DEV bool load_key_point(aff& P, const aff* pts, u32 i) {
     return m != 0xFFFFFFFFu;
 }
 
+// 256-bit little-endian integer from 32 bytes of any alignment.
+UFS_DEV void load_le32_bytes(u32 r[8], const u8* p) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const u8* q = p + 4 * i;
+        r[i] = (u32)q[0] | ((u32)q[1] << 8) | ((u32)q[2] << 16) | ((u32)q[3] << 24);
+    }
+}
+
 // One ECDSA verification of item i of batch d; returns the verdict.
 // `slot` is the lane's private table area (kSlotEntries affine entries);
 // `pts`, when not null, holds the batch's decompressed public keys.
-UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts) {
+// opaque: the batch is of kKindEcdsaOpaque -- r and s are little-endian
+// 256-bit integers, strict in [1,n) (src/cpu/src/impl/ufsecp_ecdsa.cpp:105-117,
+// 128-140), and high-S is accepted: the reference normalises s to n - s,
+// which leaves x(R) unchanged.  Items are d.stride bytes apart when the
+// stride is set (rows), else packed.
+UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+                               bool opaque) {
     u32 z_raw[8], r_raw[8], s_raw[8], x_raw[8];
-    load_be32_aligned(z_raw, d.msgs + 32ull * i);
-    load_be32_aligned(r_raw, d.sigs + 64ull * i);
-    load_be32_aligned(s_raw, d.sigs + 64ull * i + 32);
-    const u8* pk = d.pubs + 33ull * i;
+    const u8* pk;
+    if (opaque) {
+        const u64 st = d.stride;
+        const u8* m = d.msgs + (st ? st : 32ull) * i;
+        const u8* sg = d.sigs + (st ? st : 64ull) * i;
+        pk = d.pubs + (st ? st : 33ull) * i;
+        load_be32_bytes(z_raw, m);
+        load_le32_bytes(r_raw, sg);
+        load_le32_bytes(s_raw, sg + 32);
+    } else {
+        load_be32_aligned(z_raw, d.msgs + 32ull * i);
+        load_be32_aligned(r_raw, d.sigs + 64ull * i);
+        load_be32_aligned(s_raw, d.sigs + 64ull * i + 32);
+        pk = d.pubs + 33ull * i;
+    }
     const u32 prefix = pk[0];
     load_be32_bytes(x_raw, pk + 1);
     bool ok = !u256_is_zero(r_raw) && u256_lt_n(r_raw) && !u256_is_zero(s_raw) && u256_lt_n(s_raw);
     sc s, r, z;
 #pragma unroll
     for (int k = 0; k < 8; ++k) { s.v[k] = s_raw[k]; r.v[k] = r_raw[k]; z.v[k] = z_raw[k]; }
-    ok &= !sc_is_high(s);
+    if (!opaque) ok &= !sc_is_high(s);
     ok &= (prefix == 2u) | (prefix == 3u);
     aff P;
     if (pts) {
@@ -230,8 +256,8 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         const u32 i = first + lane;
         if (i >= d.count) continue;
         const u32 kind = KIND == 2 ? d.kind : (u32)KIND;
-        const bool v = kind == 0 ? ecdsa_verify_item(d, i, G, slot, pts)
-                                 : schnorr_verify_item(d, i, mid, G, slot, pts);
+        const bool v = kind == kKindSchnorr ? schnorr_verify_item(d, i, mid, G, slot, pts)
+                                            : ecdsa_verify_item(d, i, G, slot, pts, kind == kKindEcdsaOpaque);
         if (a.collect) {
             if (v) d.out[i] = 0;
         } else {

@@ -201,6 +201,62 @@ ufsecp_error_t ufsecp_gpu_schnorr_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_
     UFSECP_GPU_CATCH
 }
 
+// Argument checks in the reference's order (ufsecp_gpu_impl.cpp:340-359):
+// NULL ctx, count 0, count cap and stride, NULL buffers, then the cleared
+// output.
+ufsecp_error_t ufsecp_gpu_ecdsa_verify_opaque_rows(ufsecp_gpu_ctx* ctx, const uint8_t* rows, size_t stride,
+                                                   size_t count, uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN || stride < 129u) return UFSECP_ERR_BAD_INPUT;
+    if (!rows || !out_results) return UFSECP_ERR_NULL_ARG;
+    if (!clear_out(out_results, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    try {
+        return to_abi_clear(ctx->backend->ecdsa_verify_opaque_rows(rows, stride, count, out_results), out_results,
+                            count, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ecdsa_verify_lbtc_rows(ufsecp_gpu_ctx* ctx, const uint8_t* rows, size_t stride,
+                                                 size_t count, uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN || stride < 129u) return UFSECP_ERR_BAD_INPUT;
+    if (!rows || !out_results) return UFSECP_ERR_NULL_ARG;
+    return ufsecp_gpu_ecdsa_verify_opaque_rows(ctx, rows, stride, count, out_results);
+}
+
+ufsecp_error_t ufsecp_gpu_ecdsa_verify_lbtc_columns(ufsecp_gpu_ctx* ctx, const uint8_t* digests32,
+                                                    const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
+                                                    uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_results, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!digests32 || !pubkeys33 || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->backend->ecdsa_verify_opaque_columns(digests32, pubkeys33, sigs64, count, out_results),
+                            out_results, count, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_schnorr_verify_lbtc_columns(ufsecp_gpu_ctx* ctx, const uint8_t* digests32,
+                                                      const uint8_t* xonly32, const uint8_t* sigs64, size_t count,
+                                                      uint8_t* out_results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_results, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!digests32 || !xonly32 || !sigs64 || !out_results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->backend->schnorr_verify_batch(digests32, xonly32, sigs64, count, out_results, false),
+                            out_results, count, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
 ufsecp_error_t ufsecp_gpu_ecdsa_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
                                                const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
                                                uint8_t* key_buffer) {
@@ -274,12 +330,17 @@ ufsecp_error_t ufsecp_gpu_ext_schnorr_verify_device(ufsecp_gpu_ctx* ctx, const u
 static_assert(sizeof(ufsecp_gpu_verify_batch_t) == sizeof(ufs::BatchDesc), "batch descriptor layout");
 static_assert(offsetof(ufsecp_gpu_verify_batch_t, out_results) == offsetof(ufs::BatchDesc, out),
               "batch descriptor layout");
+static_assert(offsetof(ufsecp_gpu_verify_batch_t, row_stride) == offsetof(ufs::BatchDesc, stride),
+              "batch descriptor layout");
+static_assert(UFSECP_GPU_KIND_ECDSA_OPAQUE == ufs::kKindEcdsaOpaque, "batch kinds");
 
 static ufsecp_error_t check_batches(const ufsecp_gpu_verify_batch_t* b, size_t nb) {
     if (nb > 0 && !b) return UFSECP_ERR_NULL_ARG;
     size_t total = 0;
     for (size_t i = 0; i < nb; ++i) {
-        if (b[i].kind > 1) return UFSECP_ERR_BAD_INPUT;
+        if (b[i].kind > UFSECP_GPU_KIND_ECDSA_OPAQUE || b[i].reserved != 0) return UFSECP_ERR_BAD_INPUT;
+        if (b[i].row_stride != 0 && (b[i].kind != UFSECP_GPU_KIND_ECDSA_OPAQUE || b[i].row_stride < 129))
+            return UFSECP_ERR_BAD_INPUT;
         if (b[i].count == 0) continue;
         if (!b[i].msg_hashes32 || !b[i].pubkeys || !b[i].sigs64 || !b[i].out_results) return UFSECP_ERR_NULL_ARG;
         total += b[i].count;

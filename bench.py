@@ -185,6 +185,45 @@ def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) 
             "parity": parity, "plain_matches_persistent": same}
 
 
+def bench_lbtc(gpu, ecdsa_batch, schnorr_batch, reps: int) -> dict:
+    """libbitcoin entry points end to end from host buffers (upload, kernel,
+    download, as the reference bridge measures them): ECDSA opaque rows of
+    stride 129 and BIP-340 columns, best of `reps` wall-clock calls."""
+    import numpy as np
+    out = {"reference_published": {"ecdsa_rows_e2e_sigs_per_s": 3.56e6, "schnorr_columns_e2e_sigs_per_s": 4.53e6,
+                                   "device": "RTX 5060 Ti",
+                                   "source": "benchmarks/gpu/cuda/rtx-50xx/lbtc_batch_verify_rtx5060ti_20260530.txt"}}
+    parity = True
+    if ecdsa_batch is not None:
+        b = ecdsa_batch
+        opaque = np.concatenate([b.sigs[:, 31::-1], b.sigs[:, 63:31:-1]], axis=1)
+        rows = np.ascontiguousarray(np.concatenate([b.msgs, b.pubkeys, opaque], axis=1))
+        best = float("inf")
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            v = gpu.ecdsa_verify_opaque_rows(rows, 129, b.n)
+            best = min(best, time.perf_counter() - t0)
+        parity &= bool(np.array_equal(v, 1 - b.corrupted))
+        out["ecdsa_rows_n"] = b.n
+        out["ecdsa_rows_e2e_ms"] = round(best * 1e3, 3)
+        out["ecdsa_rows_e2e_sigs_per_s"] = b.n / best
+        del rows, opaque
+    if schnorr_batch is not None:
+        b = schnorr_batch
+        best = float("inf")
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            v = gpu.schnorr_verify_lbtc_columns(b.msgs, b.pubkeys, b.sigs)
+            best = min(best, time.perf_counter() - t0)
+        parity &= bool(np.array_equal(v, 1 - b.corrupted))
+        out["schnorr_columns_n"] = b.n
+        out["schnorr_columns_e2e_ms"] = round(best * 1e3, 3)
+        out["schnorr_columns_e2e_sigs_per_s"] = b.n / best
+    out["parity"] = parity
+    log(f"[bench] lbtc end to end: {out}")
+    return out
+
+
 def bench_msm(gpu, n: int, windows, reps: int) -> dict:
     """Pippenger MSM over n terms with bases k_i*G: c sweep, device-resident
     inputs, result checked against the closed form (sum s_i k_i) G."""
@@ -265,8 +304,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-items", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="ecdsa,mix,msm,serve",
-                    help="extra measurements after the headline: ecdsa, mix (uneven batches), msm, serve")
+    ap.add_argument("--legs", default="ecdsa,lbtc,mix,msm,serve",
+                    help="extra measurements after the headline: ecdsa, lbtc (libbitcoin rows/columns end to "
+                         "end), mix (uneven batches), msm, serve")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-items", default="4194304,2097152",
                     help="ECDSA,BIP-340 items pre-generated for the serving leg")
@@ -305,13 +345,17 @@ def main() -> int:
         cores = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(kind, batch, cores, args.cpu_items, verdicts)
         log(f"[bench] cpu baseline: {cpu}")
-    del batch, verdicts
+    del verdicts
     legs = {}
     want = [x for x in args.legs.split(",") if x]
+    ecdsa_batch = batch if kind == "ecdsa" else None
     if "ecdsa" in want and kind != "ecdsa":
-        r, _, _ = run_verify(gpu, "ecdsa", n, max(3, args.steps // 2), 1)
+        r, ecdsa_batch, _ = run_verify(gpu, "ecdsa", n, max(3, args.steps // 2), 1)
         r["valu_frac"] = r["kernel_sigs_per_s"] * W_ECDSA / mad
         legs["ecdsa"] = r
+    if "lbtc" in want:
+        legs["lbtc"] = bench_lbtc(gpu, ecdsa_batch, batch if kind == "schnorr" else None, 2)
+    del batch, ecdsa_batch
     if "mix" in want:
         legs["mix"] = bench_mix(gpu, args.mix_batches, 4096, args.mix_plain_batches, 3)
         log(f"[bench] mix: {legs['mix']}")

@@ -179,6 +179,31 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_drain(ufsecp_gpu_server* server)
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_stats(ufsecp_gpu_server* server, ufsecp_gpu_server_stats_t* out,
                                                       int reset);
 
+/* ---- reference CPU engine hook ------------------------------------------ */
+
+/* The reference engine's GPU column-verify hook
+ * (src/cpu/include/secp256k1/batch_verify.hpp:149-168), served by this
+ * library: kind 0 = opaque ECDSA columns (keys = pubkeys33, the verdicts of
+ * ufsecp_gpu_ecdsa_verify_lbtc_columns), kind 1 = BIP-340 columns (keys =
+ * xonly32).  Returns 1 (handled, all valid), 0 (handled, some invalid) or -1
+ * (declined: no device, an operational error, count > 2^26).  One device
+ * context per process, created on first use; calls are serialised. */
+UFSECP_API int ufsecp_gpu_ext_engine_columns_hook(int kind, const uint8_t* digests32, const uint8_t* keys,
+                                                  const uint8_t* sigs64, size_t count,
+                                                  uint8_t* out_results) UFSECP_NOEXCEPT;
+/* Installs the hook above into the reference engine when its installer
+ * (secp256k1::install_gpu_columns_verify_hook) is visible in the process.
+ * Runs by itself when the library loads (UFSECP_GPU_ENGINE_HOOK=0 disables
+ * that); returns 1 if installed, 0 if no engine was found. */
+UFSECP_API int ufsecp_gpu_ext_engine_hook_install(void);
+typedef struct {
+    uint64_t calls;
+    uint64_t handled;
+    uint64_t declined;
+    uint64_t items;    /* items verified on the device */
+} ufsecp_gpu_engine_hook_stats_t;
+UFSECP_API void ufsecp_gpu_ext_engine_hook_stats(ufsecp_gpu_engine_hook_stats_t* out);
+
 #ifdef __cplusplus
 }
 #endif

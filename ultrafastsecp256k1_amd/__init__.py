@@ -50,7 +50,8 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_server_create", "ufsecp_gpu_ext_server_destroy", "ufsecp_gpu_ext_server_submit",
     "ufsecp_gpu_ext_server_submit_mapped",
     "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
-    "ufsecp_gpu_ext_server_stats",
+    "ufsecp_gpu_ext_server_stats", "ufsecp_gpu_ext_engine_columns_hook", "ufsecp_gpu_ext_engine_hook_install",
+    "ufsecp_gpu_ext_engine_hook_stats",
 )
 
 
@@ -82,6 +83,20 @@ class ServerStats(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class EngineHookStats(ctypes.Structure):
+    """ufsecp_gpu_engine_hook_stats_t (include/ufsecp/ufsecp_gpu_ext.h)."""
+    _fields_ = [("calls", ctypes.c_uint64), ("handled", ctypes.c_uint64), ("declined", ctypes.c_uint64),
+                ("items", ctypes.c_uint64)]
+
+
+def engine_hook_stats(lib: Optional[ctypes.CDLL] = None) -> dict:
+    """Counters of the reference-engine column hook served by this library."""
+    lib = lib or load_library()
+    st = EngineHookStats()
+    lib.ufsecp_gpu_ext_engine_hook_stats(byref(st))
+    return {k: int(getattr(st, k)) for k, _ in EngineHookStats._fields_}
 
 
 class UfsecpError(RuntimeError):
@@ -162,6 +177,9 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_server_wait": (c_int, [vp, ctypes.c_uint64]),
         "ufsecp_gpu_ext_server_drain": (c_int, [vp]),
         "ufsecp_gpu_ext_server_stats": (c_int, [vp, vp, c_int]),
+        "ufsecp_gpu_ext_engine_columns_hook": (c_int, [c_int, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ext_engine_hook_install": (c_int, []),
+        "ufsecp_gpu_ext_engine_hook_stats": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

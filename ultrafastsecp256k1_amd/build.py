@@ -35,7 +35,7 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp",
-               "ufsecp_gpu_abi.cpp"]
+               "ufsecp_gpu_abi.cpp", "engine_hook.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -155,6 +155,28 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_schnorr_verify_collect(ufsecp_gpu_ctx* ctx,
                                                             const uint8_t* pubkeys_x32, const uint8_t* sigs64,
                                                             size_t count, uint8_t* key_buffer);
 
+/* ---- public-data key operations (libbitcoin surface) -------------------- */
+
+/* 1 per key that is a valid compressed point (prefix 02/03, x < p, on the
+ * curve), as the reference CPU ufsecp_pubkey_parse.  Replaces
+ * ufsecp_gpu_pubkey_validate (ufsecp_gpu.h:477-481). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_pubkey_validate(ufsecp_gpu_ctx* ctx, const uint8_t* pubkeys33, size_t n,
+                                                     uint8_t* results);
+
+/* 1 per x-only key with x < p and x^3 + 7 a square (lift_x succeeds).
+ * Replaces ufsecp_gpu_xonly_validate (ufsecp_gpu.h:419-423). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_xonly_validate(ufsecp_gpu_ctx* ctx, const uint8_t* keys32, size_t n,
+                                                    uint8_t* results);
+
+/* BIP-341 tweak-add check per item: Q = lift_x(internal_x, even y) + t*G;
+ * 1 iff t < n, Q is finite, x(Q) == tweaked_x and y(Q) is odd exactly when
+ * parity[i] != 0 (the reference CPU two-term ufsecp_multi_scalar_mul check,
+ * compat/libbitcoin_bridge/src/ufsecp_libbitcoin.cpp:1138-1149).  Replaces
+ * ufsecp_gpu_commitment_verify (ufsecp_gpu.h:439-446). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_commitment_verify(ufsecp_gpu_ctx* ctx, const uint8_t* internal_x32,
+                                                       const uint8_t* tweak32, const uint8_t* tweaked_x32,
+                                                       const uint8_t* parity, size_t n, uint8_t* results);
+
 /* ---- multi-scalar multiplication ---------------------------------------- */
 
 /* out_result33 = compressed sum(scalars[i] * points[i]); scalars32 N*32

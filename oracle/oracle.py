@@ -38,6 +38,9 @@ def lib() -> ctypes.CDLL:
         L.orc_schnorr_sign.argtypes = [vp, vp, vp, vp, vp]
         L.orc_verify_batch.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp, ctypes.c_int]
         L.orc_ecdsa_verify_opaque.argtypes = [vp, vp, vp]
+        L.orc_pubkey_validate.argtypes = [vp]
+        L.orc_xonly_validate.argtypes = [vp]
+        L.orc_commitment_verify.argtypes = [vp, vp, vp, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -57,6 +60,19 @@ def ecdsa_verify(msg: bytes, sig: bytes, pub33: bytes) -> int:
 def ecdsa_verify_opaque(msg: bytes, opaque_sig: bytes, pub33: bytes) -> int:
     """Opaque-signature ECDSA (little-endian r, s; high-S accepted)."""
     return lib().orc_ecdsa_verify_opaque(msg, opaque_sig, pub33)
+
+
+def pubkey_validate(pub33: bytes) -> int:
+    return lib().orc_pubkey_validate(bytes(pub33))
+
+
+def xonly_validate(x32: bytes) -> int:
+    return lib().orc_xonly_validate(bytes(x32))
+
+
+def commitment_verify(internal_x32: bytes, tweak32: bytes, tweaked_x32: bytes, parity: int) -> int:
+    """BIP-341 tweak-add check: x(lift_x(internal) + t*G) == tweaked_x with the claimed parity."""
+    return lib().orc_commitment_verify(bytes(internal_x32), bytes(tweak32), bytes(tweaked_x32), int(parity))
 
 
 def compact_to_opaque(sig64: bytes) -> bytes:

@@ -13,6 +13,12 @@
  *                      128-140, 180-189 (opaque little-endian r,s strict in
  *                      [1,n), s normalised: high-S accepted), :412-452
  *                      (per-row verdict); then the orc_ecdsa_verify core
+ *   orc_pubkey_validate, orc_xonly_validate
+ *                      ufsecp_pubkey_parse of 02/03||x (reference
+ *                      compat/libbitcoin_bridge/src/ufsecp_libbitcoin.cpp:
+ *                      1423-1427, 1531-1534; ecdsa.cpp:891-927)
+ *   orc_commitment_verify  the same file :1138-1149 (1*P + t*G by
+ *                      ufsecp_multi_scalar_mul, compressed compare)
  *   orc_schnorr_verify src/cpu/src/impl/ufsecp_ecdsa.cpp:992-1019,
  *                      schnorr.cpp:355-371 (r < p, 0 < s < n), :275-300
  *                      (lift_x, x < p), :745-806 (R = sG - eP, even y, x == r)
@@ -383,6 +389,31 @@ ORC_API int orc_schnorr_verify(const u8* msg32, const u8* sig64, const u8* px32)
     aff A;
     if (!j_to_aff(&A, R)) return 0;
     return u_cmp(A.x, r) == 0 && (A.y.v[0] & 1) == 0;
+}
+
+/* ---- public-data key operations ----------------------------------------- */
+ORC_API int orc_pubkey_validate(const u8* pub33) {
+    aff Q;
+    return pub33_parse(&Q, pub33);
+}
+
+ORC_API int orc_xonly_validate(const u8* x32) {
+    aff Q;
+    return lift(&Q, x32, 0);
+}
+
+/* BIP-341 tweak-add check as the reference bridge computes it: the two-term
+ * MSM 1*lift_x(internal, even y) + t*G (t >= n, a bad point or infinity
+ * fail), then the compressed result against (parity ? 03 : 02) || tweaked_x. */
+ORC_API int orc_commitment_verify(const u8* internal_x32, const u8* tweak32, const u8* tweaked_x32, int parity) {
+    u256 t = be_load(tweak32);
+    if (u_cmp(t, N_) >= 0) return 0;
+    aff P;
+    if (!lift(&P, internal_x32, 0)) return 0;
+    jac Q = j_mul2(t, G_(), f_small(1), j_from(P));
+    aff A;
+    if (!j_to_aff(&A, Q)) return 0;
+    return u_cmp(A.x, be_load(tweaked_x32)) == 0 && (int)(A.y.v[0] & 1) == (parity != 0);
 }
 
 /* reference CPU ufsecp_multi_scalar_mul codes: 0 OK, 5 BAD_INPUT, 3 BAD_PUBKEY, 7 ARITH */

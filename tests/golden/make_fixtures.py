@@ -25,6 +25,11 @@ Runs only in the container that holds the reference sources: it loads
   verdicts of the reference CPU ufsecp_ecdsa_verify_opaque_rows (rows of
   stride 129 and 137) and ufsecp_ecdsa_verify_opaque_batch (columns), plus
   ufsecp_ecdsa_sig_compact_to_opaque outputs that fix the byte layout.
+* ``pubdata_vectors.json`` -- compressed and x-only key validation verdicts
+  (ufsecp_pubkey_parse) and BIP-341 commitment checks: the seven BIP-341
+  key-path vectors of compat/libsecp256k1_shim/tests/test_bip341_vectors.cpp,
+  rows built as in compat/libbitcoin_bridge/tests/test_lbtc_commitment.cpp,
+  and corruptions, with the reference CPU verdicts.
 
 Usage: python tests/golden/make_fixtures.py [fixture name ...]
 """
@@ -111,6 +116,10 @@ class Ref:
         out = ctypes.create_string_buffer(n)
         rc = self.lib.ufsecp_ecdsa_verify_opaque_batch(self.ctx(), msgs, pubs, sigs, ctypes.c_size_t(n), out)
         return rc, list(out.raw)
+
+    def pubkey_parse(self, pk33: bytes) -> int:
+        out = ctypes.create_string_buffer(33)
+        return 1 if self.lib.ufsecp_pubkey_parse(self.ctx(), pk33, ctypes.c_size_t(33), out) == 0 else 0
 
     def msm(self, scalars: bytes, points: bytes, n: int):
         out = ctypes.create_string_buffer(33)
@@ -442,12 +451,106 @@ def opaque_vectors(ref: Ref) -> dict:
             "stride_checks": {"stride_128_rc": ref.verify_opaque_rows(rows129, 128, 1)[0]}}
 
 
+def pubdata_vectors(ref: Ref) -> dict:
+    """Public-data key operations with the reference CPU verdicts: key
+    validation through ufsecp_pubkey_parse, and the BIP-341 commitment
+    check as the libbitcoin bridge computes it on the CPU
+    (compat/libbitcoin_bridge/src/ufsecp_libbitcoin.cpp:1138-1149: a two-term
+    ufsecp_multi_scalar_mul 1*P + t*G, compressed result compared with
+    (parity ? 03 : 02) || tweaked_x)."""
+    n, p = N_ORDER, P_FIELD
+    b32 = lambda v: (v % (1 << 256)).to_bytes(32, "big")  # noqa: E731
+    G33 = ref.pubkey33(b32(1))
+    xs = {}
+    for k in (1, 2, 3, 7, 0xC0FFEE, n - 1, 0x1234567890ABCDEF1234567890ABCDEF):
+        xs[f"x_of_{k % 100000}G"] = ref.pubkey33(b32(k))[1:]
+    for v in range(0, 12):
+        xs[f"x_{v}"] = b32(v)
+    xs["x_eq_p"] = b32(p)
+    xs["x_p_plus_1"] = b32(p + 1)
+    xs["x_p_minus_1"] = b32(p - 1)
+    xs["x_ff"] = b"\xff" * 32
+    xs["x_gx_plus_p"] = b32(int.from_bytes(G33[1:], "big") + p) if int.from_bytes(G33[1:], "big") + p < (1 << 256) \
+        else b32(p + 2)
+    keys = []
+    for name, x in xs.items():
+        for pre in (2, 3):
+            pk = bytes([pre]) + x
+            keys.append({"name": f"{name}_{pre:02x}", "key": pk.hex(), "verdict": ref.pubkey_parse(pk)})
+    good = ref.pubkey33(b32(0xC0FFEE))
+    for pre in (0, 1, 4, 5, 6, 7, 0xFF):
+        pk = bytes([pre]) + good[1:]
+        keys.append({"name": f"prefix_{pre:02x}", "key": pk.hex(), "verdict": ref.pubkey_parse(pk)})
+    xonly = [{"name": name, "x": x.hex(), "verdict": ref.pubkey_parse(b"\x02" + x)} for name, x in xs.items()]
+
+    def commit(ix: bytes, tw: bytes):
+        rc, q = ref.msm(b32(1) + tw, b"\x02" + ix + G33, 2)
+        return (q[1:], q[0] & 1) if rc == 0 else (None, None)
+
+    def check(ix, tw, tx, par):
+        rc, q = ref.msm(b32(1) + tw, b"\x02" + ix + G33, 2)
+        return 1 if rc == 0 and q == bytes([3 if par else 2]) + tx else 0
+
+    com = []
+
+    def add(name, ix, tw, tx, par):
+        com.append({"name": name, "internal": ix.hex(), "tweak": tw.hex(), "tweaked": tx.hex(), "parity": par,
+                    "verdict": check(ix, tw, tx, par)})
+
+    src = (REF_SRC / "compat/libsecp256k1_shim/tests/test_bip341_vectors.cpp").read_text()
+    vecs = re.findall(r'\{"([0-9a-f]{64})","([0-9a-f]{64})","([0-9a-f]{64})"\}', src)
+    assert len(vecs) == 7, len(vecs)
+    for i, (ix, tw, tx) in enumerate(vecs):
+        ix, tw, tx = bytes.fromhex(ix), bytes.fromhex(tw), bytes.fromhex(tx)
+        qx, qp = commit(ix, tw)
+        assert qx == tx, ("reference MSM disagrees with BIP-341 vector", i)
+        add(f"bip341_{i}", ix, tw, tx, qp)
+        add(f"bip341_{i}_wrong_parity", ix, tw, tx, 1 - qp)
+    # rows built as in compat/libbitcoin_bridge/tests/test_lbtc_commitment.cpp:41-59
+    for i in range(40):
+        sk = bytearray(32)
+        sk[24], sk[25], sk[26], sk[31] = (i >> 24) & 0xFF, (i >> 16) & 0xFF, (i >> 8) & 0xFF, (i | 1) & 0xFF
+        ix = ref.pubkey33(bytes(sk))[1:]
+        tw = bytes((((i * 2654435761 + b * 7 + 1) >> (b % 24)) | (1 if b == 31 else 0)) & 0xFF for b in range(32))
+        qx, qp = commit(ix, tw)
+        if i % 4 == 1:
+            add(f"row_{i}_flip_x", ix, tw, bytes([qx[0] ^ 1]) + qx[1:], qp)
+        elif i % 4 == 2:
+            add(f"row_{i}_parity_byte_{2 + i}", ix, tw, qx, (2 + i) if qp else 0)
+        else:
+            add(f"row_{i}", ix, tw, qx, qp)
+    ix = ref.pubkey33(b32(0xABCDEF))[1:]
+    tw = b32(0x123456789)
+    qx, qp = commit(ix, tw)
+    add("valid", ix, tw, qx, qp)
+    add("parity_ff", ix, tw, qx, 0xFF if qp else 0)
+    add("tweak_plus_n", ix, b32(0x123456789 + n), qx, qp)
+    add("tweak_eq_n", ix, b32(n), ix, 0)
+    add("tweak_ff", ix, b"\xff" * 32, qx, qp)
+    add("tweak_zero", ix, b32(0), ix, 0)
+    add("tweak_zero_odd", ix, b32(0), ix, 1)
+    x = 5
+    while pow((x ** 3 + 7) % p, (p - 1) // 2, p) == 1:
+        x += 1
+    add("internal_off_curve", b32(x), tw, qx, qp)
+    add("internal_ge_p", b32(p + 1), tw, qx, qp)
+    # Q = infinity: internal = d*G with even y, tweak = n - d
+    d = 0x5EED5EED
+    pk = ref.pubkey33(b32(d))
+    if pk[0] == 3:
+        d = n - d
+    add("result_infinity", pk[1:], b32(n - d), pk[1:], 0)
+    add("tweak_n_minus_1", ix, b32(n - 1), *commit(ix, b32(n - 1)))
+    return {"pubkey": keys, "xonly": xonly, "commitment": com}
+
+
 def main():
     """Writes every fixture, or only those named on the command line
     (e.g. ``make_fixtures.py opaque_vectors``)."""
     ref = Ref()
     makers = {"bip340_vectors": bip340_vectors, "rfc6979_vectors": rfc6979_vectors, "edge_cases": edge_cases,
-              "workload_heads": workload_heads, "opaque_vectors": opaque_vectors}
+              "workload_heads": workload_heads, "opaque_vectors": opaque_vectors,
+              "pubdata_vectors": pubdata_vectors}
     for name in sys.argv[1:] or list(makers):
         (HERE / f"{name}.json").write_text(json.dumps(makers[name](ref), indent=1))
         print("wrote", HERE / f"{name}.json")

@@ -51,7 +51,8 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_server_submit_mapped",
     "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
     "ufsecp_gpu_ext_server_stats", "ufsecp_gpu_ext_engine_columns_hook", "ufsecp_gpu_ext_engine_hook_install",
-    "ufsecp_gpu_ext_engine_hook_stats",
+    "ufsecp_gpu_ext_engine_hook_stats", "ufsecp_gpu_pubkey_validate", "ufsecp_gpu_xonly_validate",
+    "ufsecp_gpu_commitment_verify",
 )
 
 
@@ -145,6 +146,9 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ecdsa_verify_lbtc_rows": (c_int, [vp, vp, c_size_t, c_size_t, vp]),
         "ufsecp_gpu_ecdsa_verify_lbtc_columns": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_lbtc_columns": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_pubkey_validate": (c_int, [vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_xonly_validate": (c_int, [vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_commitment_verify": (c_int, [vp, vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_ecdsa_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
@@ -314,6 +318,32 @@ class UfsecpGpu:
         rc = self._lib.ufsecp_gpu_schnorr_verify_lbtc_columns(self._ctx, _buf(digests32), _buf(xonly32),
                                                               _buf(sigs64), n, _buf(out))
         self._check("ufsecp_gpu_schnorr_verify_lbtc_columns", rc)
+        return out
+
+    # -- public-data key operations -------------------------------------------
+    def pubkey_validate(self, pubkeys33, n: Optional[int] = None):
+        import numpy as np
+        n = _nitems(pubkeys33, 33) if n is None else n
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_pubkey_validate(self._ctx, _buf(pubkeys33), n, _buf(out))
+        self._check("ufsecp_gpu_pubkey_validate", rc)
+        return out
+
+    def xonly_validate(self, keys32, n: Optional[int] = None):
+        import numpy as np
+        n = _nitems(keys32, 32) if n is None else n
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_xonly_validate(self._ctx, _buf(keys32), n, _buf(out))
+        self._check("ufsecp_gpu_xonly_validate", rc)
+        return out
+
+    def commitment_verify(self, internal_x32, tweak32, tweaked_x32, parity, n: Optional[int] = None):
+        import numpy as np
+        n = _nitems(internal_x32, 32) if n is None else n
+        out = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_commitment_verify(self._ctx, _buf(internal_x32), _buf(tweak32),
+                                                    _buf(tweaked_x32), _buf(parity), n, _buf(out))
+        self._check("ufsecp_gpu_commitment_verify", rc)
         return out
 
     def msm(self, scalars32, points33, n: Optional[int] = None) -> bytes:

@@ -300,6 +300,75 @@ GpuError RocmBackend::schnorr_verify_device(const uint8_t* d_msgs, const uint8_t
     return GpuError::Ok;
 }
 
+template <class Launch>
+GpuError RocmBackend::keyop_host(const char* what, size_t n, uint8_t* out, const uint8_t* const* in,
+                                 const size_t* bytes, int nin, Launch&& launch) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    if (n == 0) {
+        clear_error();
+        return GpuError::Ok;
+    }
+    if (n > UINT32_MAX) return fail(GpuError::BadInput, "batch count too large");
+    hipError_t e = hipSetDevice(device_);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    const uint8_t* d[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < nin; ++k) {
+        if ((e = d_in_[k].ensure(bytes[k])) != hipSuccess) return fail(GpuError::Memory, "hipMalloc(inputs)", e);
+        if ((e = hipMemcpyAsync(d_in_[k].ptr, in[k], bytes[k], hipMemcpyHostToDevice, stream_)) != hipSuccess)
+            return fail(GpuError::Memory, "copy inputs to device", e);
+        d[k] = (const uint8_t*)d_in_[k].ptr;
+    }
+    if ((e = d_out_.ensure(n)) != hipSuccess) return fail(GpuError::Memory, "hipMalloc(results)", e);
+    uint8_t* dout = (uint8_t*)d_out_.ptr;
+    unsigned grid = (unsigned)((n + 255) / 256);
+    const unsigned cap = (unsigned)cus_ * 8u;
+    if (grid > cap) grid = cap;
+    if ((e = launch(d, (uint32_t)n, dout, grid)) != hipSuccess) return fail(GpuError::Launch, what, e);
+    if ((e = hipMemcpyAsync(out, dout, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+        return fail(GpuError::Memory, "copy results to host", e);
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return fail(GpuError::Backend, "synchronize", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
+GpuError RocmBackend::pubkey_validate(const uint8_t* pk33, size_t n, uint8_t* out) {
+    const uint8_t* in[1] = {pk33};
+    const size_t bytes[1] = {n * 33};
+    return keyop_host("pubkey validate launch", n, out, in, bytes, 1,
+                      [&](const uint8_t* const* d, uint32_t cnt, uint8_t* res, unsigned grid) {
+                          return launch_pubkey_validate(d[0], cnt, res, grid, stream_);
+                      });
+}
+
+GpuError RocmBackend::xonly_validate(const uint8_t* x32, size_t n, uint8_t* out) {
+    const uint8_t* in[1] = {x32};
+    const size_t bytes[1] = {n * 32};
+    return keyop_host("x-only validate launch", n, out, in, bytes, 1,
+                      [&](const uint8_t* const* d, uint32_t cnt, uint8_t* res, unsigned grid) {
+                          return launch_xonly_validate(d[0], cnt, res, grid, stream_);
+                      });
+}
+
+// internal_x and tweak in two buffers; tweaked_x and the parity bytes share
+// the third (parity after the n x-coordinates).
+GpuError RocmBackend::commitment_verify(const uint8_t* internal_x32, const uint8_t* tweak32,
+                                        const uint8_t* tweaked_x32, const uint8_t* parity, size_t n, uint8_t* out) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    hipError_t e = hipSetDevice(device_);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    if ((e = d_in_[2].ensure(n * 33)) != hipSuccess) return fail(GpuError::Memory, "hipMalloc(inputs)", e);
+    if (n && (e = hipMemcpyAsync((uint8_t*)d_in_[2].ptr + n * 32, parity, n, hipMemcpyHostToDevice, stream_)) !=
+                 hipSuccess)
+        return fail(GpuError::Memory, "copy inputs to device", e);
+    const uint8_t* in[3] = {internal_x32, tweak32, tweaked_x32};
+    const size_t bytes[3] = {n * 32, n * 32, n * 32};
+    return keyop_host("commitment verify launch", n, out, in, bytes, 3,
+                      [&](const uint8_t* const* d, uint32_t cnt, uint8_t* res, unsigned grid) {
+                          return launch_commitment_verify(d[0], d[1], d[2], d[2] + (size_t)cnt * 32, cnt, res,
+                                                          gdesc_, grid, stream_);
+                      });
+}
+
 // Why a caller's batch descriptor is unusable, or nullptr.
 static const char* batch_desc_error(const BatchDesc& d) {
     if (d.kind > kKindEcdsaOpaque) return "batch kind must be 0 (ECDSA), 1 (BIP-340) or 2 (opaque ECDSA)";

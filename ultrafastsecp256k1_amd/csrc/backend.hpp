@@ -86,6 +86,13 @@ public:
     GpuError ecdsa_verify_opaque_rows(const uint8_t* rows, size_t stride, size_t count, uint8_t* out);
     GpuError ecdsa_verify_opaque_columns(const uint8_t* msgs, const uint8_t* pubs33, const uint8_t* sigs,
                                          size_t count, uint8_t* out);
+    // Public-data key operations of the libbitcoin surface (host buffers,
+    // synchronous): compressed-key and x-only-key validation, BIP-341
+    // tweak-add commitment check.
+    GpuError pubkey_validate(const uint8_t* pk33, size_t n, uint8_t* out);
+    GpuError xonly_validate(const uint8_t* x32, size_t n, uint8_t* out);
+    GpuError commitment_verify(const uint8_t* internal_x32, const uint8_t* tweak32, const uint8_t* tweaked_x32,
+                               const uint8_t* parity, size_t n, uint8_t* out);
 
     // Device-pointer variants (asynchronous on `st`, or the context stream).
     GpuError ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,
@@ -117,6 +124,11 @@ public:
 private:
     GpuError verify_host(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
                          size_t count, uint8_t* out, bool collect);
+    // Copies `nin` host inputs to device buffers, runs `launch` on the context
+    // stream and copies the n verdict bytes back (the public-data ops).
+    template <class Launch>
+    GpuError keyop_host(const char* what, size_t n, uint8_t* out, const uint8_t* const* in, const size_t* bytes,
+                        int nin, Launch&& launch);
     // One batch through the persistent kernel (single mode).
     hipError_t launch_single(const BatchDesc& d, bool collect, hipStream_t st);
     // Verify launches share the lane slots and the work counter: a launch on

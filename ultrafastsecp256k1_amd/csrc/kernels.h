@@ -122,6 +122,11 @@ hipError_t launch_key_resolve(const BatchDesc* descs, const TileDesc* tiles, uin
 // insert them into the cache.
 hipError_t launch_key_decompress(const BatchDesc* descs, const void* const* batch_pts, KeyCacheDesc cache,
                                  const uint32_t* miss, const uint32_t* miss_count, unsigned grid, hipStream_t st);
+// Public-data key operations (kernels_pubdata.hip): one verdict byte per item.
+hipError_t launch_pubkey_validate(const uint8_t* pk33, uint32_t n, uint8_t* res, unsigned grid, hipStream_t st);
+hipError_t launch_xonly_validate(const uint8_t* x32, uint32_t n, uint8_t* res, unsigned grid, hipStream_t st);
+hipError_t launch_commitment_verify(const uint8_t* ix, const uint8_t* tw, const uint8_t* tx, const uint8_t* par,
+                                    uint32_t n, uint8_t* res, const GcombDesc& g, unsigned grid, hipStream_t st);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,

@@ -363,4 +363,45 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
     out = acc;
 }
 
+// u*G from the fixed-base comb alone (any u < n, u = 0 gives infinity): the
+// odd representative of u (n - u with the sign folded into the entries when u
+// is even), the top entry, then one entry per window.  A mixed addition that
+// meets a == b doubles instead.
+UFS_DEV void ecmult_gen(jac& out, const gcomb& G, const sc& u) {
+    const u32 N[8] = {UFS_N_LIMBS};
+    sc ku;
+    const bool ueven = (u.v[0] & 1u) == 0;
+    {
+        u32 bw = 0, t[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t[i] = subc(N[i], u.v[i], bw, bw);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ku.v[i] = ueven ? t[i] : u.v[i];
+    }
+    const u32 fg = ueven ? kAddNeg : 0u;
+    const int half = 1 << (G.w - 1);
+    jac acc;
+    {
+        aff e;
+        load_aff(e, G.tab + (size_t)G.nwin * half);
+        fe_cond_neg(e.y, ueven);
+        jac_from_aff(acc, e);
+    }
+    for (int i = 0; i < G.nwin; ++i) {
+        u32 idx;
+        bool ng;
+        odd_digit(ku.v, i, G.w, idx, ng);
+        jac t;
+        bool nd;
+        jac_madd_mem_body(t, acc, G.tab + (size_t)i * half + idx, ng ? (fg ^ kAddNeg) : fg, nd);
+        if (__any(nd)) {
+            jac d;
+            jac_dbl(d, acc);
+            if (nd) t = d;
+        }
+        acc = t;
+    }
+    out = acc;
+}
+
 }  // namespace ufs

@@ -257,6 +257,47 @@ ufsecp_error_t ufsecp_gpu_schnorr_verify_lbtc_columns(ufsecp_gpu_ctx* ctx, const
     UFSECP_GPU_CATCH
 }
 
+// Public-data key operations; argument checks as ufsecp_gpu_impl.cpp:560-640.
+ufsecp_error_t ufsecp_gpu_pubkey_validate(ufsecp_gpu_ctx* ctx, const uint8_t* pubkeys33, size_t n,
+                                          uint8_t* results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(results, n, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!pubkeys33 || !results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->backend->pubkey_validate(pubkeys33, n, results), results, n, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_xonly_validate(ufsecp_gpu_ctx* ctx, const uint8_t* keys32, size_t n, uint8_t* results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(results, n, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!keys32 || !results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->backend->xonly_validate(keys32, n, results), results, n, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_commitment_verify(ufsecp_gpu_ctx* ctx, const uint8_t* internal_x32, const uint8_t* tweak32,
+                                            const uint8_t* tweaked_x32, const uint8_t* parity, size_t n,
+                                            uint8_t* results) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(results, n, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!internal_x32 || !tweak32 || !tweaked_x32 || !parity || !results) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi_clear(ctx->backend->commitment_verify(internal_x32, tweak32, tweaked_x32, parity, n, results),
+                            results, n, 1);
+    }
+    UFSECP_GPU_CATCH
+}
+
 ufsecp_error_t ufsecp_gpu_ecdsa_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
                                                const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
                                                uint8_t* key_buffer) {

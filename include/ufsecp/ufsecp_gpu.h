@@ -177,6 +177,18 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_commitment_verify(ufsecp_gpu_ctx* ctx, cons
                                                        const uint8_t* tweak32, const uint8_t* tweaked_x32,
                                                        const uint8_t* parity, size_t n, uint8_t* results);
 
+/* ---- public-key recovery ------------------------------------------------ */
+
+/* Per item, the compressed public key recovered from (msg_hash, compact
+ * big-endian r||s, recovery id 0..3); a failed item gets 33 zero bytes and
+ * out_valid[i] = 0.  Any recovery id outside 0..3 -> BAD_INPUT for the whole
+ * call.  Per-item results equal the reference CPU ufsecp_ecdsa_recover
+ * (impl/ufsecp_ecdsa.cpp:710-739): r, s strict in [1,n), ids 2/3 need
+ * r < p - n.  Replaces ufsecp_gpu_ecrecover_batch (ufsecp_gpu.h:596-603). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecrecover_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                     const uint8_t* sigs64, const int* recids, size_t count,
+                                                     uint8_t* out_pubkeys33, uint8_t* out_valid);
+
 /* ---- multi-scalar multiplication ---------------------------------------- */
 
 /* out_result33 = compressed sum(scalars[i] * points[i]); scalars32 N*32

@@ -38,6 +38,7 @@ def lib() -> ctypes.CDLL:
         L.orc_schnorr_sign.argtypes = [vp, vp, vp, vp, vp]
         L.orc_verify_batch.argtypes = [ctypes.c_int, vp, vp, vp, sz, vp, ctypes.c_int]
         L.orc_ecdsa_verify_opaque.argtypes = [vp, vp, vp]
+        L.orc_ecdsa_recover.argtypes = [vp, vp, ctypes.c_int, vp]
         L.orc_pubkey_validate.argtypes = [vp]
         L.orc_xonly_validate.argtypes = [vp]
         L.orc_commitment_verify.argtypes = [vp, vp, vp, ctypes.c_int]
@@ -60,6 +61,13 @@ def ecdsa_verify(msg: bytes, sig: bytes, pub33: bytes) -> int:
 def ecdsa_verify_opaque(msg: bytes, opaque_sig: bytes, pub33: bytes) -> int:
     """Opaque-signature ECDSA (little-endian r, s; high-S accepted)."""
     return lib().orc_ecdsa_verify_opaque(msg, opaque_sig, pub33)
+
+
+def ecdsa_recover(msg: bytes, sig64: bytes, recid: int):
+    """(1, compressed key) or (0, 33 zero bytes)."""
+    out = ctypes.create_string_buffer(33)
+    ok = lib().orc_ecdsa_recover(bytes(msg), bytes(sig64), int(recid), out)
+    return ok, out.raw
 
 
 def pubkey_validate(pub33: bytes) -> int:

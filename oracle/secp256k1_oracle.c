@@ -13,6 +13,9 @@
  *                      128-140, 180-189 (opaque little-endian r,s strict in
  *                      [1,n), s normalised: high-S accepted), :412-452
  *                      (per-row verdict); then the orc_ecdsa_verify core
+ *   orc_ecdsa_recover  src/cpu/src/impl/ufsecp_ecdsa.cpp:710-739 (strict
+ *                      r, s; recid 0..3), recovery.cpp:158-235 (r + n only
+ *                      when r < p - n, lift with parity, Q finite)
  *   orc_pubkey_validate, orc_xonly_validate
  *                      ufsecp_pubkey_parse of 02/03||x (reference
  *                      compat/libbitcoin_bridge/src/ufsecp_libbitcoin.cpp:
@@ -389,6 +392,35 @@ ORC_API int orc_schnorr_verify(const u8* msg32, const u8* sig64, const u8* px32)
     aff A;
     if (!j_to_aff(&A, R)) return 0;
     return u_cmp(A.x, r) == 0 && (A.y.v[0] & 1) == 0;
+}
+
+/* ---- public-key recovery ------------------------------------------------- */
+/* Q = (-z/r) G + (s/r) R, R = lift(r [+ n for ids 2, 3], parity = id & 1);
+ * 1 and the compressed key on success, 0 and 33 zero bytes on failure. */
+ORC_API int orc_ecdsa_recover(const u8* msg32, const u8* sig64, int recid, u8* out33) {
+    memset(out33, 0, 33);
+    if (recid < 0 || recid > 3) return 0;
+    u256 r = be_load(sig64), s = be_load(sig64 + 32);
+    if (u_zero(r) || u_cmp(r, N_) >= 0 || u_zero(s) || u_cmp(s, N_) >= 0) return 0;
+    u256 rx = r;
+    if (recid & 2) {
+        u256 pmn;
+        u_sub(&pmn, P_, N_);
+        if (u_cmp(r, pmn) >= 0) return 0;
+        u_add(&rx, r, N_);
+    }
+    u8 xb[32];
+    be_store(xb, rx);
+    aff R;
+    if (!lift(&R, xb, recid & 1)) return 0;
+    u256 ri = s_inv(r);
+    u256 u1 = s_mul(s_neg(s_from_bytes(msg32)), ri), u2 = s_mul(s, ri);
+    jac Q = j_mul2(u1, G_(), u2, j_from(R));
+    aff A;
+    if (!j_to_aff(&A, Q)) return 0;
+    out33[0] = (u8)(2 | (A.y.v[0] & 1));
+    be_store(out33 + 1, A.x);
+    return 1;
 }
 
 /* ---- public-data key operations ----------------------------------------- */

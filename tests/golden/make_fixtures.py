@@ -30,6 +30,10 @@ Runs only in the container that holds the reference sources: it loads
   key-path vectors of compat/libsecp256k1_shim/tests/test_bip341_vectors.cpp,
   rows built as in compat/libbitcoin_bridge/tests/test_lbtc_commitment.cpp,
   and corruptions, with the reference CPU verdicts.
+* ``recover_vectors.json`` -- ECDSA public-key recovery items (every recovery
+  id of reference-signed recoverable signatures, high-S twins, out-of-range
+  r/s, crafted ids 2/3 items with R.x = r + n) with the reference CPU
+  ufsecp_ecdsa_recover results.
 
 Usage: python tests/golden/make_fixtures.py [fixture name ...]
 """
@@ -120,6 +124,17 @@ class Ref:
     def pubkey_parse(self, pk33: bytes) -> int:
         out = ctypes.create_string_buffer(33)
         return 1 if self.lib.ufsecp_pubkey_parse(self.ctx(), pk33, ctypes.c_size_t(33), out) == 0 else 0
+
+    def sign_recoverable(self, msg: bytes, d: bytes):
+        out = ctypes.create_string_buffer(64)
+        rid = ctypes.c_int(-1)
+        assert self.lib.ufsecp_ecdsa_sign_recoverable(self.ctx(), msg, d, out, ctypes.byref(rid)) == 0
+        return out.raw, rid.value
+
+    def recover(self, msg: bytes, sig: bytes, recid: int):
+        out = ctypes.create_string_buffer(33)
+        rc = self.lib.ufsecp_ecdsa_recover(self.ctx(), msg, sig, ctypes.c_int(recid), out)
+        return (1, out.raw) if rc == 0 else (0, bytes(33))
 
     def msm(self, scalars: bytes, points: bytes, n: int):
         out = ctypes.create_string_buffer(33)
@@ -544,13 +559,66 @@ def pubdata_vectors(ref: Ref) -> dict:
     return {"pubkey": keys, "xonly": xonly, "commitment": com}
 
 
+def recover_vectors(ref: Ref) -> list:
+    """ECDSA public-key recovery items with the reference CPU
+    ufsecp_ecdsa_recover result (flag, 33-byte key or zeros): signatures from
+    ufsecp_ecdsa_sign_recoverable under every recovery id, high-S twins,
+    out-of-range r/s, and crafted ids 2/3 items whose R has x = r + n."""
+    n, p = N_ORDER, P_FIELD
+    b32 = lambda v: (v % (1 << 256)).to_bytes(32, "big")  # noqa: E731
+    items = []
+
+    def add(name, msg, sig, recid):
+        ok, key = ref.recover(msg, sig, recid)
+        items.append({"name": name, "msg": msg.hex(), "sig": sig.hex(), "recid": recid, "valid": ok,
+                      "key": key.hex()})
+
+    for i in range(12):
+        d = hashlib.sha256(b"recover-key-%d" % i).digest()
+        msg = hashlib.sha256(b"recover-msg-%d" % i).digest()
+        sig, rid = ref.sign_recoverable(msg, d)
+        pub = ref.pubkey33(d)
+        for recid in range(4):
+            add(f"sig_{i}_recid_{recid}{'_own' if recid == rid else ''}", msg, sig, recid)
+        assert items[-4 + rid]["key"] == pub.hex() and items[-4 + rid]["valid"] == 1
+        r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+        add(f"sig_{i}_high_s_recid_{rid ^ 1}", msg, b32(r) + b32(n - s), rid ^ 1)
+    d = b32(0xFEEDFACE)
+    msg = hashlib.sha256(b"recover-edge").digest()
+    sig, rid = ref.sign_recoverable(msg, d)
+    r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+    add("r_zero", msg, b32(0) + b32(s), rid)
+    add("s_zero", msg, b32(r) + b32(0), rid)
+    add("r_eq_n", msg, b32(n) + b32(s), rid)
+    add("s_eq_n", msg, b32(r) + b32(n), rid)
+    add("r_ff", msg, b"\xff" * 32 + b32(s), rid)
+    add("msg_zero", b32(0), sig, rid)
+    add("msg_ff", b"\xff" * 32, sig, rid)
+    # ids 2/3: R.x = r + n must be a curve x below p; find small r with x^3 + 7 a square
+    found = 0
+    k = 1
+    while found < 3:
+        x = n + k
+        if x < p and pow((x ** 3 + 7) % p, (p - 1) // 2, p) == 1:
+            for recid in (2, 3):
+                add(f"high_x_r_{k}_recid_{recid}", msg, b32(k) + b32(s), recid)
+                add(f"high_x_r_{k}_recid_{recid - 2}", msg, b32(k) + b32(s), recid - 2)
+            found += 1
+        k += 1
+    pmn = p - n
+    add("ids23_r_eq_p_minus_n", msg, b32(pmn) + b32(s), 2)
+    add("ids23_r_p_minus_n_minus_1", msg, b32(pmn - 1) + b32(s), 3)
+    add("ids23_ordinary_r", msg, sig, 2 | (rid & 1))
+    return items
+
+
 def main():
     """Writes every fixture, or only those named on the command line
     (e.g. ``make_fixtures.py opaque_vectors``)."""
     ref = Ref()
     makers = {"bip340_vectors": bip340_vectors, "rfc6979_vectors": rfc6979_vectors, "edge_cases": edge_cases,
               "workload_heads": workload_heads, "opaque_vectors": opaque_vectors,
-              "pubdata_vectors": pubdata_vectors}
+              "pubdata_vectors": pubdata_vectors, "recover_vectors": recover_vectors}
     for name in sys.argv[1:] or list(makers):
         (HERE / f"{name}.json").write_text(json.dumps(makers[name](ref), indent=1))
         print("wrote", HERE / f"{name}.json")

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from ec_math import G, N, P, compress, lift_x, mul
+from ec_math import add as add_pts
 from oracle import oracle as orc
 from ultrafastsecp256k1_amd import workload as wl
 
@@ -223,3 +224,34 @@ def test_commitment_closed_form():
         assert orc.commitment_verify(ix, tw, tx, 1 - (Q[1] & 1)) == 0
         assert orc.xonly_validate(ix) == 1
         assert orc.pubkey_validate(compress(mul(d))) == 1
+
+
+def test_recover_vectors(golden):
+    """Public-key recovery against the reference CPU ufsecp_ecdsa_recover."""
+    for c in golden["recover_vectors"]:
+        ok, key = orc.ecdsa_recover(bytes.fromhex(c["msg"]), bytes.fromhex(c["sig"]), c["recid"])
+        assert (ok, key.hex()) == (c["valid"], c["key"]), c["name"]
+
+
+def test_recover_closed_form():
+    """Sign with the oracle (pinned by RFC 6979 vectors), recover with the
+    right id: the signer's key; the key recovered under id ^ 1 is the
+    textbook (-z/r) G + (s/r) (-R)."""
+    rnd = random.Random(23)
+    for _ in range(12):
+        d = rnd.randrange(1, N)
+        msg = rnd.randbytes(32)
+        sig, k = orc.ecdsa_sign(msg, d.to_bytes(32, "big"))
+        R = mul(int.from_bytes(k, "big"))
+        r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+        s_raw = pow(int.from_bytes(k, "big"), -1, N) * (int.from_bytes(msg, "big") % N + r * d) % N
+        Ry = R[1] if s == s_raw else P - R[1]          # the signer normalised s: R -> -R
+        recid = (Ry & 1) | (2 if R[0] >= N else 0)
+        ok, key = orc.ecdsa_recover(msg, sig, recid)
+        assert ok == 1 and key == compress(mul(d))
+        z = int.from_bytes(msg, "big") % N
+        ri = pow(r, -1, N)
+        other = (R[0], P - Ry)
+        want = add_pts(mul((-z * ri) % N), mul(s * ri % N, other))
+        ok2, key2 = orc.ecdsa_recover(msg, sig, recid ^ 1)
+        assert ok2 == 1 and key2 == compress(want)

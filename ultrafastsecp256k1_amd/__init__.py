@@ -52,7 +52,7 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
     "ufsecp_gpu_ext_server_stats", "ufsecp_gpu_ext_engine_columns_hook", "ufsecp_gpu_ext_engine_hook_install",
     "ufsecp_gpu_ext_engine_hook_stats", "ufsecp_gpu_pubkey_validate", "ufsecp_gpu_xonly_validate",
-    "ufsecp_gpu_commitment_verify",
+    "ufsecp_gpu_commitment_verify", "ufsecp_gpu_ecrecover_batch",
 )
 
 
@@ -149,6 +149,7 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_pubkey_validate": (c_int, [vp, vp, c_size_t, vp]),
         "ufsecp_gpu_xonly_validate": (c_int, [vp, vp, c_size_t, vp]),
         "ufsecp_gpu_commitment_verify": (c_int, [vp, vp, vp, vp, vp, c_size_t, vp]),
+        "ufsecp_gpu_ecrecover_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
         "ufsecp_gpu_ecdsa_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
@@ -345,6 +346,18 @@ class UfsecpGpu:
                                                     _buf(tweaked_x32), _buf(parity), n, _buf(out))
         self._check("ufsecp_gpu_commitment_verify", rc)
         return out
+
+    def ecrecover_batch(self, msgs, sigs64, recids, count: Optional[int] = None):
+        """(keys (count, 33) uint8, valid (count,) uint8); recids int32 0..3."""
+        import numpy as np
+        n = _nitems(msgs, 32) if count is None else count
+        rec = np.ascontiguousarray(np.asarray(recids, dtype=np.int32))
+        keys = np.zeros((n, 33), dtype=np.uint8)
+        valid = np.zeros(n, dtype=np.uint8)
+        rc = self._lib.ufsecp_gpu_ecrecover_batch(self._ctx, _buf(msgs), _buf(sigs64), _buf(rec), n, _buf(keys),
+                                                  _buf(valid))
+        self._check("ufsecp_gpu_ecrecover_batch", rc)
+        return keys, valid
 
     def msm(self, scalars32, points33, n: Optional[int] = None) -> bytes:
         n = _nitems(scalars32, 32) if n is None else n

@@ -369,6 +369,44 @@ GpuError RocmBackend::commitment_verify(const uint8_t* internal_x32, const uint8
                       });
 }
 
+// Inputs in d_in_[0..2] (digests, signatures, recovery ids), keys and flags
+// in d_out_ (33 n bytes of keys, then n flags).  The kernel takes the lane
+// slots, so it is ordered after any verify launch like another verify launch.
+GpuError RocmBackend::ecrecover_batch(const uint8_t* msgs, const uint8_t* sigs, const int* recids, size_t n,
+                                      uint8_t* out33, uint8_t* valid) {
+    if (!ready_) return fail(GpuError::Device, "context not initialised");
+    if (n == 0) {
+        clear_error();
+        return GpuError::Ok;
+    }
+    if (n > UINT32_MAX) return fail(GpuError::BadInput, "batch count too large");
+    hipError_t e = hipSetDevice(device_);
+    if (e != hipSuccess) return fail(GpuError::Device, "hipSetDevice", e);
+    if ((e = d_in_[0].ensure(n * 32)) != hipSuccess || (e = d_in_[1].ensure(n * 64)) != hipSuccess ||
+        (e = d_in_[2].ensure(n * sizeof(int))) != hipSuccess || (e = d_out_.ensure(n * 34)) != hipSuccess)
+        return fail(GpuError::Memory, "hipMalloc(recover buffers)", e);
+    e = hipMemcpyAsync(d_in_[0].ptr, msgs, n * 32, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[1].ptr, sigs, n * 64, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[2].ptr, recids, n * sizeof(int), hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy inputs to device", e);
+    uint8_t* dk = (uint8_t*)d_out_.ptr;
+    uint8_t* dv = dk + n * 33;
+    unsigned grid = (unsigned)((n + kVerifyBlock - 1) / kVerifyBlock);
+    if (grid > verify_grid_[0]) grid = verify_grid_[0];
+    e = verify_begin(stream_);
+    if (e == hipSuccess)
+        e = launch_ecrecover((const uint8_t*)d_in_[0].ptr, (const uint8_t*)d_in_[1].ptr, (const int*)d_in_[2].ptr,
+                             (uint32_t)n, dk, dv, d_slots_.ptr, gdesc_, grid, stream_);
+    if (e == hipSuccess) e = verify_end(stream_);
+    if (e != hipSuccess) return fail(GpuError::Launch, "recover launch", e);
+    e = hipMemcpyAsync(out33, dk, n * 33, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(valid, dv, n, hipMemcpyDeviceToHost, stream_);
+    if (e != hipSuccess) return fail(GpuError::Memory, "copy results to host", e);
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return fail(GpuError::Backend, "synchronize", e);
+    clear_error();
+    return GpuError::Ok;
+}
+
 // Why a caller's batch descriptor is unusable, or nullptr.
 static const char* batch_desc_error(const BatchDesc& d) {
     if (d.kind > kKindEcdsaOpaque) return "batch kind must be 0 (ECDSA), 1 (BIP-340) or 2 (opaque ECDSA)";

@@ -93,6 +93,9 @@ public:
     GpuError xonly_validate(const uint8_t* x32, size_t n, uint8_t* out);
     GpuError commitment_verify(const uint8_t* internal_x32, const uint8_t* tweak32, const uint8_t* tweaked_x32,
                                const uint8_t* parity, size_t n, uint8_t* out);
+    // ECDSA public-key recovery (host buffers, synchronous); recids must be 0..3.
+    GpuError ecrecover_batch(const uint8_t* msgs, const uint8_t* sigs, const int* recids, size_t n,
+                             uint8_t* out33, uint8_t* valid);
 
     // Device-pointer variants (asynchronous on `st`, or the context stream).
     GpuError ecdsa_verify_device(const uint8_t* d_msgs, const uint8_t* d_pubs33, const uint8_t* d_sigs,

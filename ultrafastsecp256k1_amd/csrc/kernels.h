@@ -127,6 +127,11 @@ hipError_t launch_pubkey_validate(const uint8_t* pk33, uint32_t n, uint8_t* res,
 hipError_t launch_xonly_validate(const uint8_t* x32, uint32_t n, uint8_t* res, unsigned grid, hipStream_t st);
 hipError_t launch_commitment_verify(const uint8_t* ix, const uint8_t* tw, const uint8_t* tx, const uint8_t* par,
                                     uint32_t n, uint8_t* res, const GcombDesc& g, unsigned grid, hipStream_t st);
+// ECDSA public-key recovery (kernels_recover.hip): 33-byte compressed key
+// (zeros on failure) and a flag per item; `slots` as for launch_verify, for
+// at least grid * kVerifyBlock lanes.
+hipError_t launch_ecrecover(const uint8_t* msgs, const uint8_t* sigs, const int* recids, uint32_t n, uint8_t* out33,
+                            uint8_t* valid, void* slots, const GcombDesc& g, unsigned grid, hipStream_t st);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,

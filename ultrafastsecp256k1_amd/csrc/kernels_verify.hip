@@ -45,13 +45,6 @@ UFS_DEV void load_be32_bytes(u32 r[8], const u8* p) {
     }
 }
 
-UFS_DEV void set_generator(aff& g) {
-    const fe GX = {{UFS_GX_LIMBS}};
-    const fe GY = {{UFS_GY_LIMBS}};
-    g.x = GX;
-    g.y = GY;
-}
-
 // x(R) == r (mod n) test without inversion: r*Z^2 == X, or (r+n)*Z^2 == X
 // when r + n < p.
 UFS_DEV bool ecdsa_xcheck(const jac& R, const u32 r[8]) {

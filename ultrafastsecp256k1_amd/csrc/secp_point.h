@@ -301,6 +301,13 @@ UFS_DEV void jac_to_aff(aff& r, const jac& a) {
     fe_normalize(r.y);
 }
 
+UFS_DEV void set_generator(aff& g) {
+    const fe GX = {{UFS_GX_LIMBS}};
+    const fe GY = {{UFS_GY_LIMBS}};
+    g.x = GX;
+    g.y = GY;
+}
+
 // Decompress x (raw limbs) to a point with the requested y parity.
 // Returns false if x >= p or x^3 + 7 is not a square.
 UFS_DEV bool lift_x(aff& r, const u32 xraw[8], u32 want_odd) {

@@ -298,6 +298,31 @@ ufsecp_error_t ufsecp_gpu_commitment_verify(ufsecp_gpu_ctx* ctx, const uint8_t* 
     UFSECP_GPU_CATCH
 }
 
+// Argument checks as ufsecp_gpu_impl.cpp:756-786 (recovery ids checked for
+// the whole batch before dispatch; outputs cleared on any non-OK).
+ufsecp_error_t ufsecp_gpu_ecrecover_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32, const uint8_t* sigs64,
+                                          const int* recids, size_t count, uint8_t* out_pubkeys33,
+                                          uint8_t* out_valid) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_pubkeys33, count, 33)) return UFSECP_ERR_BAD_INPUT;
+    if (!clear_out(out_valid, count, 1)) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !sigs64 || !recids || !out_pubkeys33 || !out_valid) return UFSECP_ERR_NULL_ARG;
+    for (size_t i = 0; i < count; ++i)
+        if (recids[i] < 0 || recids[i] > 3) return UFSECP_ERR_BAD_INPUT;
+    try {
+        const ufsecp_error_t r =
+            to_abi(ctx->backend->ecrecover_batch(msg_hashes32, sigs64, recids, count, out_pubkeys33, out_valid));
+        if (r != UFSECP_OK) {
+            (void)clear_out(out_pubkeys33, count, 33);
+            (void)clear_out(out_valid, count, 1);
+        }
+        return r;
+    }
+    UFSECP_GPU_CATCH
+}
+
 ufsecp_error_t ufsecp_gpu_ecdsa_verify_collect(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
                                                const uint8_t* pubkeys33, const uint8_t* sigs64, size_t count,
                                                uint8_t* key_buffer) {

@@ -179,6 +179,17 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_drain(ufsecp_gpu_server* server)
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_server_stats(ufsecp_gpu_server* server, ufsecp_gpu_server_stats_t* out,
                                                       int reset);
 
+/* BIP-340 batch verification by a random linear combination (the
+ * reference CPU engine's schnorr_batch_verify, src/cpu/src/batch_verify.cpp:
+ * 324-424, on the device): one MSM of 2N + 1 terms with fresh random weights
+ * per call.  *all_valid = 1 when every signature would verify under
+ * ufsecp_gpu_schnorr_verify_batch, 0 otherwise (a wrong 1 has probability
+ * about 2^-128).  No per-item verdicts: on 0, run the per-item call to find
+ * the invalid ones.  count == 0 -> OK with *all_valid = 0. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_schnorr_batch_check(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                            const uint8_t* pubkeys_x32, const uint8_t* sigs64,
+                                                            size_t count, int* all_valid);
+
 /* ---- reference CPU engine hook ------------------------------------------ */
 
 /* The reference engine's GPU column-verify hook

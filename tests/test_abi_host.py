@@ -108,3 +108,10 @@ def test_corruption_schedule_rate():
     assert np.array_equal(tail.sigs, b.sigs[777:781])
     assert np.array_equal(tail.pubkeys, b.pubkeys[777:781])
     assert np.array_equal(tail.corrupted, b.corrupted[777:781])
+
+
+def test_batch_check_null_context():
+    lib = ufs.load_library()
+    ok = ctypes.c_int(5)
+    buf = (ctypes.c_uint8 * 128)()
+    assert lib.ufsecp_gpu_ext_schnorr_batch_check(None, buf, buf, buf, ctypes.c_size_t(1), ctypes.byref(ok)) == 1

@@ -52,7 +52,7 @@ ABI_SYMBOLS = (
     "ufsecp_gpu_ext_server_poll", "ufsecp_gpu_ext_server_wait", "ufsecp_gpu_ext_server_drain",
     "ufsecp_gpu_ext_server_stats", "ufsecp_gpu_ext_engine_columns_hook", "ufsecp_gpu_ext_engine_hook_install",
     "ufsecp_gpu_ext_engine_hook_stats", "ufsecp_gpu_pubkey_validate", "ufsecp_gpu_xonly_validate",
-    "ufsecp_gpu_commitment_verify", "ufsecp_gpu_ecrecover_batch",
+    "ufsecp_gpu_commitment_verify", "ufsecp_gpu_ecrecover_batch", "ufsecp_gpu_ext_schnorr_batch_check",
 )
 
 
@@ -150,6 +150,7 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_xonly_validate": (c_int, [vp, vp, c_size_t, vp]),
         "ufsecp_gpu_commitment_verify": (c_int, [vp, vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_ecrecover_batch": (c_int, [vp, vp, vp, vp, c_size_t, vp, vp]),
+        "ufsecp_gpu_ext_schnorr_batch_check": (c_int, [vp, vp, vp, vp, c_size_t, POINTER(c_int)]),
         "ufsecp_gpu_ecdsa_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_schnorr_verify_collect": (c_int, [vp, vp, vp, vp, c_size_t, vp]),
         "ufsecp_gpu_msm": (c_int, [vp, vp, vp, c_size_t, vp]),
@@ -346,6 +347,15 @@ class UfsecpGpu:
                                                     _buf(tweaked_x32), _buf(parity), n, _buf(out))
         self._check("ufsecp_gpu_commitment_verify", rc)
         return out
+
+    def schnorr_batch_check(self, msgs, pubkeys_x32, sigs64, count: Optional[int] = None) -> bool:
+        """True when every BIP-340 signature verifies (random linear combination, one MSM)."""
+        n = _nitems(msgs, 32) if count is None else count
+        ok = c_int(-1)
+        rc = self._lib.ufsecp_gpu_ext_schnorr_batch_check(self._ctx, _buf(msgs), _buf(pubkeys_x32), _buf(sigs64), n,
+                                                          byref(ok))
+        self._check("ufsecp_gpu_ext_schnorr_batch_check", rc)
+        return ok.value == 1
 
     def ecrecover_batch(self, msgs, sigs64, recids, count: Optional[int] = None):
         """(keys (count, 33) uint8, valid (count,) uint8); recids int32 0..3."""

@@ -35,8 +35,8 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "kernels_pubdata.hip", "kernels_recover.hip",
-               "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp",
-               "ufsecp_gpu_abi.cpp", "engine_hook.cpp"]
+               "kernels_rlc.hip", "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp", "ufsecp_gpu_abi.cpp",
+               "engine_hook.cpp", "batch_rlc.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

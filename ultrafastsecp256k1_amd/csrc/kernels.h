@@ -132,6 +132,17 @@ hipError_t launch_commitment_verify(const uint8_t* ix, const uint8_t* tw, const 
 // at least grid * kVerifyBlock lanes.
 hipError_t launch_ecrecover(const uint8_t* msgs, const uint8_t* sigs, const int* recids, uint32_t n, uint8_t* out33,
                             uint8_t* valid, void* slots, const GcombDesc& g, unsigned grid, hipStream_t st);
+// BIP-340 batch check by a random linear combination (kernels_rlc.hip):
+// writes the 2n + 1 MSM terms (scalars 32 B big-endian, points 33 B
+// compressed) for n signatures and the weight seed; *err |= 1 when some s is
+// 0 or >= n.  `partial` holds rlc_partial_words(n) words.
+struct RlcSeed {
+    uint32_t w[8];   // big-endian words of the 32-byte seed
+};
+hipError_t launch_rlc_terms(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, uint32_t n,
+                            const RlcSeed& seed, const uint32_t* mid, uint8_t* scalars, uint8_t* points,
+                            uint32_t* partial, uint32_t* err, hipStream_t st);
+uint32_t rlc_partial_words(uint32_t n);
 hipError_t launch_build_gcomb(void* table, int w, int nwin, hipStream_t st);
 hipError_t launch_bip340_midstate(uint32_t* out, hipStream_t st);
 hipError_t launch_selftest(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t count,

@@ -481,7 +481,8 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
 MsmEngine::~MsmEngine() {
     DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_, &keys_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
-                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_};
+                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_,
+                           &rlc_in_, &rlc_terms_, &rlc_part_};
     for (DeviceBuffer* b : all) b->release();
 }
 

@@ -62,6 +62,12 @@ public:
     // Device buffers; asynchronous on st.  *d_status receives the ABI code.
     GpuError msm_device(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                         int32_t* d_status, int c, hipStream_t st);
+    // BIP-340 batch check by a random linear combination, one MSM of 2n + 1
+    // terms per chunk (host buffers, synchronous): *all_valid = 1 when every
+    // signature verifies, 0 otherwise (a wrong 1 has probability about 2^-128
+    // per call).  Defined in batch_rlc.cpp.
+    GpuError schnorr_batch_check(const uint8_t* msgs, const uint8_t* pubx, const uint8_t* sigs, size_t n,
+                                 int* all_valid);
 
 private:
     GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
@@ -69,6 +75,7 @@ private:
     RocmBackend* be_;
     DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, keys_, bsum_, seg_, red_a_, red_b_, err_,
         scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_flags_, sel_count_;
+    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_;
 };
 
 }  // namespace ufs

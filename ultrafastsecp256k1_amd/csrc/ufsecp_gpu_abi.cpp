@@ -467,6 +467,21 @@ ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_s
     UFSECP_GPU_CATCH
 }
 
+ufsecp_error_t ufsecp_gpu_ext_schnorr_batch_check(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                 const uint8_t* pubkeys_x32, const uint8_t* sigs64, size_t count,
+                                                 int* all_valid) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (!all_valid) return UFSECP_ERR_NULL_ARG;
+    *all_valid = 0;
+    if (count == 0) return UFSECP_OK;
+    if (count > kMaxGpuBatchN) return UFSECP_ERR_BAD_INPUT;
+    if (!msg_hashes32 || !pubkeys_x32 || !sigs64) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->msm->schnorr_batch_check(msg_hashes32, pubkeys_x32, sigs64, count, all_valid));
+    }
+    UFSECP_GPU_CATCH
+}
+
 ufsecp_error_t ufsecp_gpu_ext_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** d_ptr_out) {
     if (!ctx || !d_ptr_out) return UFSECP_ERR_NULL_ARG;
     *d_ptr_out = nullptr;

@@ -49,15 +49,33 @@ def test_msm_infinity_is_arith_error(gpu):
     assert rc == UFSECP_ERR_ARITH and out == b"\x00" * 33
 
 
-@pytest.mark.parametrize("c", [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
+@pytest.mark.parametrize("c", [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_msm_every_window_width(gpu, c):
     """Explicit window widths: every c gives the closed-form point, including
-    the top window's LDS counting (257 - c(W-1) <= 8 bits) and the small
+    the top window's LDS counting (256 - c(W-1) <= 8 bits) and the small
     bucket counts of c < 4."""
     n = 777
     sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + c)
     e = wl.msm_closed_form_scalar(sc, ks)
     rc, out = gpu.msm_rc(sc, pts, n, c)
+    assert rc == UFSECP_OK
+    assert out == compress(mul(e))
+
+
+@pytest.mark.parametrize("c", [2, 5, 8, 11, 13, 16, 17, 20])
+def test_msm_top_window_extremes(gpu, c):
+    """Scalars whose signed recoding carries into the top window up to its
+    largest unsigned digit (n - 1, n - 2, 2^255 + 2^254, all-ones low bits),
+    beside zero and one: the top window's second bucket group is used."""
+    N = wl.N_ORDER
+    specials = [N - 1, N - 2, (1 << 255) + (1 << 254), (1 << 256) - (1 << 200) - 1 - ((1 << 256) - N),
+                (1 << 128) - 1, 1, 0, N // 2, N // 2 + 1]
+    specials = [v % N for v in specials]
+    ks = [3 + 7 * i for i in range(len(specials))]
+    sc = b"".join(v.to_bytes(32, "big") for v in specials)
+    pts = b"".join(compress(mul(k)) for k in ks)
+    e = sum(v * k for v, k in zip(specials, ks)) % N
+    rc, out = gpu.msm_rc(sc, pts, len(ks), c)
     assert rc == UFSECP_OK
     assert out == compress(mul(e))
 

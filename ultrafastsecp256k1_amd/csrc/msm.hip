@@ -32,9 +32,17 @@ UFS_DEV u32 msm_bits(const u32 k[8], int pos, int w) {
     return (u32)(v >> sh) & ((1u << w) - 1u);
 }
 
-// Signed digit of window w given the running carry: |d| <= 2^(c-1).
-UFS_DEV int msm_digit(const u32 k[8], int w, int c, u32& carry) {
+// Digit of window w given the running carry.  Windows below the top one
+// are signed, |d| <= 2^(c-1), carrying into the next; the top window (W - 1,
+// tb = 256 - c(W-1) bits) takes its bits plus the carry unsigned, a value in
+// [0, 2^tb], so no extra carry window is needed (a carry window would put
+// about half of all terms into a single bucket).
+UFS_DEV int msm_digit(const u32 k[8], int w, int c, int W, u32& carry) {
     const u32 raw = msm_bits(k, w * c, c) + carry;
+    if (w == W - 1) {
+        carry = 0;
+        return (int)raw;
+    }
     const u32 half = 1u << (c - 1);
     if (raw > half) {
         carry = 1;
@@ -44,18 +52,21 @@ UFS_DEV int msm_digit(const u32 k[8], int w, int c, u32& carry) {
     return (int)raw;
 }
 
-// Counting sort of the (window, |digit|) bucket ids (counted in
-// k_msm_prepare, placed by k_msm_scatter).  Windows below the top
-// one have 2^(c-1) uniformly used buckets and count with plain global atomics.
-// The top window has only 2^top_bits buckets (top_bits = 257 - c(W-1)) that
-// receive a large share of all terms; their counters share a few cache lines,
-// so per-term global atomics on them serialise (measured 2-6 ms at 2^20
-// terms).  When top_bits <= 8 the top window is counted in LDS per block and
-// each block reserves its range with one global atomic per bucket.
-constexpr int kTermsPerThread = 8;
+// Counting sort of the bucket ids w * 2^(c-1) + |d| - 1 (counted in
+// k_msm_prepare, placed by k_msm_scatter).  Windows below the top one have
+// 2^(c-1) uniformly used buckets and count with plain global atomics.  The top
+// window's 2^tb buckets (up to 2^c, so it spans two groups of 2^(c-1)) are
+// few when tb is small and then receive a large share of all terms; their
+// counters share a few cache lines, so per-term global atomics on them
+// serialise (measured 2-6 ms at 2^20 terms).  When tb <= 8 the top window is
+// counted in LDS per block and each block reserves its range with one global
+// atomic per bucket.
+constexpr int kTermsPerThread = 2;
 constexpr int kTopLdsBits = 8;
+constexpr int kScatterGroup = 8;
+constexpr u32 kXcds = 8;   // XCDs of an MI355X; blocks are dealt to them round-robin
 
-UFS_DEV int msm_top_bits(int c, int W) { return 257 - c * (W - 1); }
+UFS_DEV int msm_top_bits(int c, int W) { return 256 - c * (W - 1); }
 
 // Parse + decompress every term (scalar < n, strict SEC1 point; the first
 // failing term decides the error, reported as 2i or 2i+1 through atomicMin)
@@ -105,7 +116,7 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
             scl[i] = s;
             u32 carry = 0;
             for (int w = 0; w < W; ++w) {
-                const int d = msm_digit(k, w, c, carry);
+                const int d = msm_digit(k, w, c, W, carry);
                 if (d == 0) continue;
                 const u32 m = (u32)(d < 0 ? -d : d) - 1u;
                 if (lds && w == W - 1) {
@@ -123,19 +134,31 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
     }
 }
 
-// Scatter of (term | sign) into bucket order; top-window positions come from
-// per-block LDS ranks plus one reserved range per bucket and block.
+// Sorted entry: bucket id in the high word, term index | sign bit in the low
+// word, written with one 8-byte store (one write transaction per scattered
+// entry instead of one per array).
+UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg) {
+    return ((u64)bucket << 32) | (u64)(term | (neg ? 0x80000000u : 0u));
+}
+
+// Scatter of (bucket, term | sign) into bucket order; top-window positions
+// come from per-block LDS ranks plus one reserved range per bucket and block.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* __restrict__ entries,
-              u32* __restrict__ keys, int c, int W) {
+k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u64* __restrict__ entries, int c,
+              int W) {
     __shared__ u32 tcnt[1 << kTopLdsBits];
     __shared__ u32 tbase[1 << kTopLdsBits];
     const u32 B = 1u << (c - 1);
     const int tb = msm_top_bits(c, W);
     const bool lds = tb <= kTopLdsBits;
     const u32 nkeys = 1u << (tb < kTopLdsBits ? tb : kTopLdsBits);
-    const size_t step = (size_t)gridDim.x * blockDim.x * kTermsPerThread;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x * kTermsPerThread; base < n; base += step) {
+    // Blocks are dealt to the XCDs round-robin, so block b runs on XCD b % 8:
+    // it handles the windows w = b % 8 (mod 8) for its share of the terms, and
+    // every cursor of a window is then updated from one XCD only.
+    const int wfirst = (int)(blockIdx.x % kXcds);
+    const u32 slice = blockIdx.x / kXcds, nslices = gridDim.x / kXcds;
+    const size_t step = (size_t)nslices * blockDim.x * kTermsPerThread;
+    for (size_t base = (size_t)slice * blockDim.x * kTermsPerThread; base < n; base += step) {
         if (lds) {
             for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) tcnt[t] = 0;
             __syncthreads();
@@ -150,18 +173,37 @@ k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* 
             if (i >= n) continue;
             const sc s = scl[i];
             u32 carry = 0;
-            for (int w = 0; w < W; ++w) {
-                const int d = msm_digit(s.v, w, c, carry);
-                if (d == 0) continue;
-                const u32 b = w * B + (u32)(d < 0 ? -d : d) - 1u;
-                if (lds && w == W - 1) {
-                    dtop[j] = d;
-                    rank[j] = atomicAdd(&tcnt[b - w * B], 1u);
-                } else {
-                    const u32 pos = atomicAdd(&cursor[b], 1u);
-                    entries[pos] = (u32)i | (d < 0 ? 0x80000000u : 0u);
-                    keys[pos] = b;
+            // groups of kScatterGroup windows: the digits first, then all the
+            // cursor reservations back to back (in flight together), then the
+            // stores; only this block's windows are placed
+            for (int w0 = 0; w0 < W; w0 += kScatterGroup) {
+                u32 bk[kScatterGroup], pos[kScatterGroup];
+                bool act[kScatterGroup], ng[kScatterGroup];
+#pragma unroll
+                for (int q = 0; q < kScatterGroup; ++q) {
+                    const int w = w0 + q;
+                    act[q] = false;
+                    ng[q] = false;
+                    bk[q] = 0;
+                    if (w < W) {
+                        const int d = msm_digit(s.v, w, c, W, carry);
+                        if (d != 0 && (w % kXcds) == wfirst) {
+                            bk[q] = w * B + (u32)(d < 0 ? -d : d) - 1u;
+                            ng[q] = d < 0;
+                            if (lds && w == W - 1) {
+                                dtop[j] = d;
+                                rank[j] = atomicAdd(&tcnt[bk[q] - w * B], 1u);
+                            } else {
+                                act[q] = true;
+                            }
+                        }
+                    }
                 }
+#pragma unroll
+                for (int q = 0; q < kScatterGroup; ++q) pos[q] = act[q] ? atomicAdd(&cursor[bk[q]], 1u) : 0u;
+#pragma unroll
+                for (int q = 0; q < kScatterGroup; ++q)
+                    if (act[q]) entries[pos[q]] = msm_entry(bk[q], (u32)i, ng[q]);
             }
         }
         if (lds) {
@@ -178,8 +220,7 @@ k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u32* 
                 const u32 m = (u32)(d < 0 ? -d : d) - 1u;
                 const u32 pos = tbase[m] + rank[j];
                 const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
-                entries[pos] = (u32)i | (d < 0 ? 0x80000000u : 0u);
-                keys[pos] = (W - 1) * B + m;
+                entries[pos] = msm_entry((W - 1) * B + m, (u32)i, d < 0);
             }
             __syncthreads();
         }
@@ -204,9 +245,8 @@ __global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p,
 // by the higher levels, so every lane does at most K additions whatever the
 // bucket sizes (the top window's buckets can hold half of all terms).
 __global__ void __launch_bounds__(kMsmBlock)
-k_seg0(const aff* __restrict__ pts, const u32* __restrict__ entries, const u32* __restrict__ keys,
-       const u32* __restrict__ total_p, u32 K, u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys,
-       jac* __restrict__ opart) {
+k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* __restrict__ total_p, u32 K,
+       u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= max_chunks) return;
     const u32 E = *total_p;
@@ -217,21 +257,21 @@ k_seg0(const aff* __restrict__ pts, const u32* __restrict__ entries, const u32* 
     u32 hk = kEmpty, tk = kEmpty;
     if (beg < E) {
         const u32 end = min(beg + K, E);
-        const bool before = beg > 0 && keys[beg - 1] == keys[beg];
-        const bool after = end < E && keys[end] == keys[end - 1];
-        u32 cur = keys[beg];
+        u64 ent = entries[beg];
+        const bool before = beg > 0 && (u32)(entries[beg - 1] >> 32) == (u32)(ent >> 32);
+        const bool after = end < E && (u32)(entries[end] >> 32) == (u32)(entries[end - 1] >> 32);
+        u32 cur = (u32)(ent >> 32);
         bool first = true;
         jac acc;
         jac_set_inf(acc);
-        u32 ent = entries[beg];
-        aff nxt = pts[ent & 0x7fffffffu];
+        aff nxt = pts[(u32)ent & 0x7fffffffu];
         for (u32 e = beg; e < end; ++e) {
-            const u32 k = keys[e];
+            const u32 k = (u32)(ent >> 32);
             aff P = nxt;
-            const bool neg = (ent >> 31) != 0;
+            const bool neg = ((u32)ent >> 31) != 0;
             if (e + 1 < end) {
                 ent = entries[e + 1];
-                nxt = pts[ent & 0x7fffffffu];
+                nxt = pts[(u32)ent & 0x7fffffffu];
             }
             if (k != cur) {
                 if (first && before) { hk = cur; hv = acc; } else { bsum[cur] = acc; }
@@ -301,6 +341,63 @@ k_seg(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* _
     opart[2 * j + 1] = tv;
 }
 
+// Run-aligned pieces over the compacted level-0 partials (bucket order):
+// a run is the partials of one bucket; a piece is up to kPieceLen consecutive
+// partials of one run.  A lane per piece sums it; a piece that is a whole run
+// completes its bucket, the pieces of longer runs go on to the chunked
+// levels.  Level-0 runs are short (a bucket spans two or three chunks) except
+// for buckets holding a large share of all terms, so this one step usually
+// finishes every bucket with two or three additions per lane.
+constexpr u32 kPieceLen = 16;
+
+// rs[i] = i where partial i starts a run, else 0 (max-scanned into the run
+// start of every partial).
+__global__ void __launch_bounds__(kMsmBlock)
+k_run_heads(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys, u32 S,
+            u32* __restrict__ rs) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const u32 E = *count_p;
+    u32 v = 0;
+    if (i < E && (i == 0 || ikeys[idx[i]] != ikeys[idx[i - 1]])) v = i;
+    rs[i] = v;
+}
+
+__global__ void __launch_bounds__(kMsmBlock)
+k_piece_flags(const u32* __restrict__ rstart, const u32* __restrict__ count_p, u32 S, u8* __restrict__ flags) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    flags[i] = (i < *count_p && (i - rstart[i]) % kPieceLen == 0) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kMsmBlock)
+k_piece_reduce(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys,
+               const jac* __restrict__ ipart, const u32* __restrict__ pieces, const u32* __restrict__ npieces_p,
+               const u32* __restrict__ rstart, u32 S, jac* __restrict__ bsum, u32* __restrict__ okeys,
+               jac* __restrict__ opart) {
+    const u32 p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S) return;
+    const u32 P = *npieces_p;
+    if (p >= P) {
+        okeys[p] = kEmpty;
+        return;
+    }
+    const u32 E = *count_p;
+    const u32 beg = pieces[p];
+    const u32 end = p + 1 < P ? pieces[p + 1] : E;
+    const u32 key = ikeys[idx[beg]];
+    jac acc = ipart[idx[beg]];
+    for (u32 j = beg + 1; j < end; ++j) jac_add(acc, acc, ipart[idx[j]]);
+    const bool whole = rstart[beg] == beg && (end == E || ikeys[idx[end]] != key);
+    if (whole) {
+        bsum[key] = acc;
+        okeys[p] = kEmpty;
+    } else {
+        okeys[p] = key;
+        opart[p] = acc;
+    }
+}
+
 // r = m * a for a small non-negative integer m (double-and-add).
 UFS_DEV void jac_mul_small(jac& r, const jac& a, u32 m) {
     jac acc;
@@ -318,11 +415,13 @@ UFS_DEV void jac_mul_small(jac& r, const jac& a, u32 m) {
 
 // Lane (w, s): P = sum_{j<L} (s*L + j + 1) * B[w][s*L + j].
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 W, u32 B, u32 L) {
+k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 G, u32 B, u32 L) {
     const u32 T = B / L;
     const u32 id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= W * T) return;
+    if (id >= G * T) return;
     const u32 w = id / T, s = id % T;
+    // group G - 1 holds magnitudes B + 1 .. 2B of the top window
+    const u32 off = s * L + (w == G - 1 ? B : 0u);
     const jac* bb = bsum + (size_t)w * B + (size_t)s * L;
     jac R, S;
     jac_set_inf(R);
@@ -332,7 +431,7 @@ k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 W, u32 B
         jac_add(S, S, R);
     }
     jac M;
-    jac_mul_small(M, R, s * L);
+    jac_mul_small(M, R, off);
     jac_add(S, S, M);
     seg[id] = S;
 }
@@ -371,7 +470,8 @@ __global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u3
         *status = (e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */;
         return;
     }
-    jac acc = wsum[W - 1];
+    jac acc;
+    jac_add(acc, wsum[W - 1], wsum[W]);   // the top window's two bucket groups
     for (int w = (int)W - 2; w >= 0; --w) {
         for (u32 d = 0; d < c; ++d) jac_dbl(acc, acc);
         jac_add(acc, acc, wsum[w]);
@@ -422,18 +522,23 @@ static unsigned sort_grid(uint32_t n, unsigned max_blocks) {
     return b < max_blocks ? b : max_blocks;
 }
 
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
-                              const MsmPlan& p, unsigned max_blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_scatter, dim3(sort_grid(n, max_blocks)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
-                       cursor, entries, keys, p.c, p.windows);
+// The scatter grid: kXcds blocks (one per window residue) per slice of terms.
+static unsigned scatter_grid(uint32_t n, unsigned max_blocks) {
+    return sort_grid(n, max_blocks / kXcds > 0 ? max_blocks / kXcds : 1) * kXcds;
+}
+
+hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint64_t* entries, const MsmPlan& p,
+                              unsigned max_blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_msm_scatter, dim3(scatter_grid(n, max_blocks)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
+                       cursor, entries, p.c, p.windows);
     return hipGetLastError();
 }
 
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st) {
     const uint32_t T = (uint32_t)(p.buckets / p.seg_len);
-    const uint32_t lanes = (uint32_t)p.windows * T;
+    const uint32_t lanes = (uint32_t)p.groups * T;
     hipLaunchKernelGGL(k_msm_segments, dim3(blocks_for(lanes, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                       (const jac*)bsum, (jac*)seg, (uint32_t)p.windows, (uint32_t)p.buckets, (uint32_t)p.seg_len);
+                       (const jac*)bsum, (jac*)seg, (uint32_t)p.groups, (uint32_t)p.buckets, (uint32_t)p.seg_len);
     return hipGetLastError();
 }
 
@@ -472,17 +577,18 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
     if (c <= 0) c = msm_default_window(n);
     MsmPlan p;
     p.c = c;
-    p.windows = (257 + c - 1) / c;
+    p.windows = (256 + c - 1) / c;
+    p.groups = p.windows + 1;
     p.buckets = 1 << (c - 1);
     p.seg_len = p.buckets < 8 ? p.buckets : 8;
     return p;
 }
 
 MsmEngine::~MsmEngine() {
-    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_, &keys_,
+    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
                            &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_,
-                           &rlc_in_, &rlc_terms_, &rlc_part_};
+                           &rlc_in_, &rlc_terms_, &rlc_part_, &run_};
     for (DeviceBuffer* b : all) b->release();
 }
 
@@ -490,8 +596,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
                             int32_t* d_status, int c, hipStream_t st) {
     if (c != 0 && (c < 2 || c > 20)) return be_->fail(GpuError::BadInput, "MSM window width out of range");
     const MsmPlan p = msm_plan(n, c, &be_->tuning());
-    const size_t nbk = (size_t)p.windows * p.buckets;
-    const size_t nseg = (size_t)p.windows * (p.buckets / p.seg_len);
+    const size_t nbk = (size_t)p.groups * p.buckets;
+    const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac);
     const size_t max_entries = n * (size_t)p.windows;
     const uint32_t K = kSegChunk;
@@ -500,28 +606,38 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     hipError_t e;
     if ((e = pts_.ensure(n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(n * sizeof(sc))) != hipSuccess ||
         (e = counts_.ensure((nbk + 1) * 4)) != hipSuccess || (e = offsets_.ensure((nbk + 1) * 4)) != hipSuccess ||
-        (e = entries_.ensure(max_entries * 4)) != hipSuccess || (e = keys_.ensure(max_entries * 4)) != hipSuccess ||
+        (e = entries_.ensure(max_entries * 8)) != hipSuccess ||
         (e = bsum_.ensure(nbk * jsz)) != hipSuccess || (e = seg_.ensure(nseg * jsz)) != hipSuccess ||
         (e = red_a_.ensure(nseg * jsz)) != hipSuccess || (e = red_b_.ensure(nseg * jsz)) != hipSuccess ||
         (e = err_.ensure(4)) != hipSuccess || (e = slot_keys_[0].ensure(slots0 * 4)) != hipSuccess ||
         (e = slot_keys_[1].ensure(slots0 * 4)) != hipSuccess || (e = slot_part_[0].ensure(slots0 * jsz)) != hipSuccess ||
         (e = slot_part_[1].ensure(slots0 * jsz)) != hipSuccess || (e = sel_idx_.ensure(slots0 * 4)) != hipSuccess ||
-        (e = sel_flags_.ensure(slots0)) != hipSuccess || (e = sel_count_.ensure(16)) != hipSuccess)
+        (e = sel_flags_.ensure(slots0)) != hipSuccess || (e = sel_count_.ensure(16)) != hipSuccess ||
+        (e = run_.ensure(slots0 * 12)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     uint32_t* counts = (uint32_t*)counts_.ptr;
     uint32_t* offsets = (uint32_t*)offsets_.ptr;
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
-    const unsigned sort_blocks = cus * 2, prep_blocks = cus * 4;
+    const unsigned sort_blocks = cus * 8, prep_blocks = cus * 4;
     uint32_t* err = (uint32_t*)err_.ptr;
     uint32_t* sel_idx = (uint32_t*)sel_idx_.ptr;
     uint32_t* sel_count = (uint32_t*)sel_count_.ptr;
-    size_t tmp_bytes = 0, sel_bytes = 0;
+    uint32_t* run_heads = (uint32_t*)run_.ptr;
+    uint32_t* run_start = run_heads + slots0;
+    uint32_t* pieces = run_start + slots0;
+    uint32_t* npieces = sel_count + 1;
+    size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, offsets, (int)(nbk + 1), st);
     if (e == hipSuccess)
         e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
                                           (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)slots0,
+                                              st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "scan size query", e);
     if (sel_bytes > tmp_bytes) tmp_bytes = sel_bytes;
+    if (max_bytes > tmp_bytes) tmp_bytes = max_bytes;
+    sel_bytes = max_bytes = tmp_bytes;
     if ((e = scan_tmp_.ensure(tmp_bytes + 16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
 
@@ -534,8 +650,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket scan", e);
     e = hipMemcpyAsync(counts, offsets, (nbk + 1) * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return be_->fail(GpuError::Memory, "MSM cursor copy", e);
-    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint32_t*)entries_.ptr, (uint32_t*)keys_.ptr, p,
-                                sort_blocks, st)) != hipSuccess)
+    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint64_t*)entries_.ptr, p, sort_blocks, st)) !=
+        hipSuccess)
         return be_->fail(GpuError::Launch, "MSM scatter", e);
     // bucket sums: empty buckets stay at infinity; level 0 over the sorted
     // entries (total = offsets[nbk]), then chunked reductions of the compacted
@@ -543,11 +659,39 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, (jac*)bsum_.ptr,
                        (uint32_t)nbk);
     hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts_.ptr,
-                       (const uint32_t*)entries_.ptr, (const uint32_t*)keys_.ptr, (const uint32_t*)(offsets + nbk),
-                       K, (uint32_t)chunks0, (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
+                       (const uint64_t*)entries_.ptr, (const uint32_t*)(offsets + nbk), K, (uint32_t)chunks0,
+                       (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
+    // run-aligned pieces: most buckets complete here; the pieces of long runs
+    // (in slot set 1) continue through the chunked levels below
+    {
+        const size_t S = slots0;
+        hipLaunchKernelGGL(k_seg_flags, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)slot_keys_[0].ptr, (uint32_t)S, (uint8_t*)sel_flags_.ptr);
+        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
+                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)S, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
+        hipLaunchKernelGGL(k_run_heads, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,
+                           (uint32_t)S, run_heads);
+        e = hipcub::DeviceScan::InclusiveScan(scan_tmp_.ptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)S,
+                                              st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM run starts", e);
+        hipLaunchKernelGGL(k_piece_flags, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)run_start, (const uint32_t*)sel_count, (uint32_t)S,
+                           (uint8_t*)sel_flags_.ptr);
+        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
+                                          (const uint8_t*)sel_flags_.ptr, pieces, npieces, (int)S, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM piece list", e);
+        hipLaunchKernelGGL(k_piece_reduce, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,
+                           (const jac*)slot_part_[0].ptr, (const uint32_t*)pieces, (const uint32_t*)npieces,
+                           (const uint32_t*)run_start, (uint32_t)S, (jac*)bsum_.ptr, (uint32_t*)slot_keys_[1].ptr,
+                           (jac*)slot_part_[1].ptr);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM pieces", e);
+    }
     size_t slots = slots0;
-    int lv = 0;
+    int lv = 1;
     for (;;) {
         const size_t chunks = (slots + K - 1) / K;
         hipLaunchKernelGGL(k_seg_flags, dim3(blocks_for(slots, kMsmBlock)), dim3(kMsmBlock), 0, st,
@@ -573,7 +717,7 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     int which = 0;
     while (len > 1) {
         const uint32_t out_len = (len + kReduceBlock - 1) / kReduceBlock;
-        e = launch_msm_reduce(cur, bufs[which], (uint32_t)p.windows, len, out_len, st);
+        e = launch_msm_reduce(cur, bufs[which], (uint32_t)p.groups, len, out_len, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM reduce", e);
         cur = bufs[which];
         which ^= 1;

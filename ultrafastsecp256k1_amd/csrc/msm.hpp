@@ -7,13 +7,15 @@
 //   1. parse + decompress: scalars < n and strict SEC1 points checked per term
 //      (first failing term decides the error, as in the reference CPU
 //      ufsecp_multi_scalar_mul, impl/ufsecp_taproot.cpp:957-991);
-//   2. signed c-bit digits (W = ceil(257/c) windows, |d| <= 2^(c-1));
+//   2. c-bit digits in W = ceil(256/c) windows: signed, |d| <= 2^(c-1), below
+//      the top window, unsigned (bits + carry, up to 2^c) in the top one;
 //      histogram of (window, |d|) bucket ids, exclusive scan, scatter of
-//      (term index | sign) into bucket-sorted order;
+//      (bucket, term index | sign) into bucket-sorted order;
 //   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
-//      entries (mixed additions), with chunk-boundary partials merged in a
-//      few compaction + reduction levels, so no lane adds more than 32 terms
-//      even when one bucket holds half of all terms;
+//      entries (mixed additions); the chunk-boundary partials of a bucket are
+//      summed by one lane per run of at most 16 (usually the last step), and
+//      longer runs go through chunked reduction levels, so no lane adds more
+//      than 32 terms even when one bucket holds half of all terms;
 //   4. per-window weighted sum sum_b (b+1) B_b by segments (running sums plus
 //      one small scalar multiplication per segment) and a block tree;
 //   5. Horner combination of the windows, affine normalisation, compression.
@@ -31,8 +33,9 @@ namespace ufs {
 
 struct MsmPlan {
     int c;          // window width
-    int windows;    // ceil(257 / c)
-    int buckets;    // 2^(c-1) per window
+    int windows;    // ceil(256 / c)
+    int groups;     // windows + 1 bucket groups: the top window spans two
+    int buckets;    // 2^(c-1) per group
     int seg_len;    // buckets per reduction segment
 };
 
@@ -44,8 +47,8 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
                               uint32_t* counts, const MsmPlan& p, uint32_t* err, unsigned max_blocks,
                               hipStream_t st);
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint32_t* entries, uint32_t* keys,
-                              const MsmPlan& p, unsigned max_blocks, hipStream_t st);
+hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint64_t* entries, const MsmPlan& p,
+                              unsigned max_blocks, hipStream_t st);
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
                              hipStream_t st);
@@ -73,9 +76,9 @@ private:
     GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                      int32_t* d_status, int c, hipStream_t st);
     RocmBackend* be_;
-    DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, keys_, bsum_, seg_, red_a_, red_b_, err_,
+    DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, bsum_, seg_, red_a_, red_b_, err_,
         scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_flags_, sel_count_;
-    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_;
+    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_;
 };
 
 }  // namespace ufs

@@ -4,7 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "msm.hpp"
-#include "secp_point.h"
+#include "secp_ecmult.h"
 
 namespace ufs {
 
@@ -66,11 +66,16 @@ constexpr int kTopLdsBits = 8;
 constexpr int kScatterGroup = 8;
 constexpr u32 kXcds = 8;   // XCDs of an MI355X; blocks are dealt to them round-robin
 
-UFS_DEV int msm_top_bits(int c, int W) { return 256 - c * (W - 1); }
+// Scalars are split by GLV (k = k1 + lambda k2, |k1|, |k2| < 2^128): every
+// term becomes two of 128 bits, on P and phi(P) = (beta x, y) = lambda P.
+constexpr int kScalarBits = 128;
+
+UFS_DEV int msm_top_bits(int c, int W) { return kScalarBits - c * (W - 1); }
 
 // Parse + decompress every term (scalar < n, strict SEC1 point; the first
-// failing term decides the error, reported as 2i or 2i+1 through atomicMin)
-// and count its bucket ids.  The counting atomics overlap the square roots.
+// failing term decides the error, reported as 2i or 2i+1 through atomicMin),
+// split it into the two GLV terms 2i: (|k1|, +-P) and 2i+1: (|k2|, +-phi(P)),
+// and count their bucket ids.  The counting atomics overlap the square roots.
 constexpr int kPrepTermsPerThread = 4;
 
 __global__ void __launch_bounds__(kMsmBlock)
@@ -109,20 +114,47 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
             } else if (!pok) {
                 atomicMin(err, 2u * (u32)i + 1u);
             }
-            pts[i] = P;
-            sc s;
+            sc s, k1, k2;
+            bool n1, n2;
 #pragma unroll
             for (int l = 0; l < 8; ++l) s.v[l] = k[l];
-            scl[i] = s;
-            u32 carry = 0;
-            for (int w = 0; w < W; ++w) {
-                const int d = msm_digit(k, w, c, W, carry);
-                if (d == 0) continue;
-                const u32 m = (u32)(d < 0 ? -d : d) - 1u;
-                if (lds && w == W - 1) {
-                    atomicAdd(&tcnt[m], 1u);
-                } else {
-                    atomicAdd(&counts[w * B + m], 1u);
+            glv_split(k1, k2, n1, n2, s);
+            // The halves are below 2^128 for every scalar below n.  A term
+            // that fails its checks (the call then returns the error) places
+            // no bucket entries: a scalar >= n would split into halves of any
+            // size, whose top-window digits would leave the bucket range.
+#pragma unroll
+            for (int l = 4; l < 8; ++l) {
+                k1.v[l] = 0;
+                k2.v[l] = 0;
+            }
+            if (!(sok && pok)) {
+                sc_set_u32(k1, 0);
+                sc_set_u32(k2, 0);
+            }
+            aff Q;
+            const fe BETA = {{UFS_BETA_LIMBS}};
+            fe_mul(Q.x, P.x, BETA);
+            fe ny;
+            fe_neg(ny, P.y);
+            Q.y = n2 ? ny : P.y;
+            if (n1) P.y = ny;
+            pts[2 * i] = P;
+            pts[2 * i + 1] = Q;
+            scl[2 * i] = k1;
+            scl[2 * i + 1] = k2;
+            for (int h = 0; h < 2; ++h) {
+                const sc& kh = h ? k2 : k1;
+                u32 carry = 0;
+                for (int w = 0; w < W; ++w) {
+                    const int d = msm_digit(kh.v, w, c, W, carry);
+                    if (d == 0) continue;
+                    const u32 m = (u32)(d < 0 ? -d : d) - 1u;
+                    if (lds && w == W - 1) {
+                        atomicAdd(&tcnt[m], 1u);
+                    } else {
+                        atomicAdd(&counts[w * B + m], 1u);
+                    }
                 }
             }
         }
@@ -577,7 +609,7 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
     if (c <= 0) c = msm_default_window(n);
     MsmPlan p;
     p.c = c;
-    p.windows = (256 + c - 1) / c;
+    p.windows = (kScalarBits + c - 1) / c;
     p.groups = p.windows + 1;
     p.buckets = 1 << (c - 1);
     p.seg_len = p.buckets < 8 ? p.buckets : 8;
@@ -599,12 +631,12 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac);
-    const size_t max_entries = n * (size_t)p.windows;
+    const size_t max_entries = 2 * n * (size_t)p.windows;   // two GLV terms per term
     const uint32_t K = kSegChunk;
     const size_t chunks0 = (max_entries + K - 1) / K;
     const size_t slots0 = 2 * chunks0;
     hipError_t e;
-    if ((e = pts_.ensure(n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(n * sizeof(sc))) != hipSuccess ||
+    if ((e = pts_.ensure(2 * n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(2 * n * sizeof(sc))) != hipSuccess ||
         (e = counts_.ensure((nbk + 1) * 4)) != hipSuccess || (e = offsets_.ensure((nbk + 1) * 4)) != hipSuccess ||
         (e = entries_.ensure(max_entries * 8)) != hipSuccess ||
         (e = bsum_.ensure(nbk * jsz)) != hipSuccess || (e = seg_.ensure(nseg * jsz)) != hipSuccess ||
@@ -650,7 +682,7 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket scan", e);
     e = hipMemcpyAsync(counts, offsets, (nbk + 1) * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return be_->fail(GpuError::Memory, "MSM cursor copy", e);
-    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)n, counts, (uint64_t*)entries_.ptr, p, sort_blocks, st)) !=
+    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)(2 * n), counts, (uint64_t*)entries_.ptr, p, sort_blocks, st)) !=
         hipSuccess)
         return be_->fail(GpuError::Launch, "MSM scatter", e);
     // bucket sums: empty buckets stay at infinity; level 0 over the sorted

@@ -7,10 +7,12 @@
 //   1. parse + decompress: scalars < n and strict SEC1 points checked per term
 //      (first failing term decides the error, as in the reference CPU
 //      ufsecp_multi_scalar_mul, impl/ufsecp_taproot.cpp:957-991);
-//   2. c-bit digits in W = ceil(256/c) windows: signed, |d| <= 2^(c-1), below
-//      the top window, unsigned (bits + carry, up to 2^c) in the top one;
-//      histogram of (window, |d|) bucket ids, exclusive scan, scatter of
-//      (bucket, term index | sign) into bucket-sorted order;
+//   2. GLV split of every scalar, k = k1 + lambda k2 with |k1|, |k2| < 2^128,
+//      into the terms (|k1|, +-P) and (|k2|, +-phi(P)), phi(P) = (beta x, y);
+//      c-bit digits of the 128-bit halves in W = ceil(128/c) windows: signed,
+//      |d| <= 2^(c-1), below the top window, unsigned (bits + carry, up to
+//      2^c) in the top one; histogram of (window, |d|) bucket ids, exclusive
+//      scan, scatter of (bucket, term index | sign) into bucket-sorted order;
 //   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
 //      entries (mixed additions); the chunk-boundary partials of a bucket are
 //      summed by one lane per run of at most 16 (usually the last step), and
@@ -33,7 +35,7 @@ namespace ufs {
 
 struct MsmPlan {
     int c;          // window width
-    int windows;    // ceil(256 / c)
+    int windows;    // ceil(128 / c) over the GLV halves
     int groups;     // windows + 1 bucket groups: the top window spans two
     int buckets;    // 2^(c-1) per group
     int seg_len;    // buckets per reduction segment

@@ -34,7 +34,7 @@ UFS_DEV u32 msm_bits(const u32 k[8], int pos, int w) {
 
 // Digit of window w given the running carry.  Windows below the top one
 // are signed, |d| <= 2^(c-1), carrying into the next; the top window (W - 1,
-// tb = 256 - c(W-1) bits) takes its bits plus the carry unsigned, a value in
+// tb = 128 - c(W-1) bits of a GLV half) takes its bits plus the carry unsigned, a value in
 // [0, 2^tb], so no extra carry window is needed (a carry window would put
 // about half of all terms into a single bucket).
 UFS_DEV int msm_digit(const u32 k[8], int w, int c, int W, u32& carry) {
@@ -311,8 +311,17 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* 
                 cur = k;
                 jac_set_inf(acc);
             }
-            if (neg) fe_neg(P.y, P.y);
-            jac_madd(acc, acc, P);
+            // inline mixed addition; a == P (possible for repeated points)
+            // doubles instead
+            jac t;
+            bool nd;
+            jac_madd_reg_body(t, acc, P, neg ? kAddNeg : 0u, nd);
+            if (__any(nd)) {
+                jac dd;
+                jac_dbl(dd, acc);
+                if (nd) t = dd;
+            }
+            acc = t;
         }
         if (first && before) { hk = cur; hv = acc; }
         else if (after) { tk = cur; tv = acc; }

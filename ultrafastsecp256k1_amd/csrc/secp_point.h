@@ -179,13 +179,11 @@ UFS_DEV void load_aff_g(aff& b, const aff* bp) {
     b.y.v[4] = q3.x; b.y.v[5] = q3.y; b.y.v[6] = q3.z; b.y.v[7] = q3.w;
 }
 
-// out = a + b for b affine in device memory (never infinity), flags as
-// above, field arithmetic inline.  The degenerate cases are reported instead
-// of handled: a == b (finite) leaves out = a and sets `need_dbl` (the caller
-// doubles); a == -b gives infinity.
-UFS_DEV void jac_madd_mem_body(jac& out, const jac& a, const aff* bp, u32 flags, bool& need_dbl) {
-    aff b;
-    load_aff_g(b, bp);
+// out = a + b for b affine (never infinity), flags as above, field
+// arithmetic inline.  The degenerate cases are reported instead of handled:
+// a == b (finite) leaves out = a and sets `need_dbl` (the caller doubles);
+// a == -b gives infinity.
+UFS_DEV void jac_madd_reg_body(jac& out, const jac& a, aff b, u32 flags, bool& need_dbl) {
     fe z1z1, u2, s2, h, hh, i4, j, rr, v, t;
     fe_sqr_i(z1z1, a.z);
     if (flags & kAddBeta) {
@@ -240,6 +238,13 @@ UFS_DEV void jac_madd_mem_body(jac& out, const jac& a, const aff* bp, u32 flags,
             }
         }
     }
+}
+
+// The same with b loaded from device memory.
+UFS_DEV void jac_madd_mem_body(jac& out, const jac& a, const aff* bp, u32 flags, bool& need_dbl) {
+    aff b;
+    load_aff_g(b, bp);
+    jac_madd_reg_body(out, a, b, flags, need_dbl);
 }
 
 // r = a + b (general Jacobian), exception-complete.

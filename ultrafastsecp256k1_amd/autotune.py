@@ -79,6 +79,14 @@ def _time_verify(gpu, d, n, kind, reps=3) -> float:
     return best
 
 
+def _smallest_within(res: dict, tol: float):
+    """The smallest setting whose throughput is within `tol` of the best: a
+    wider comb table or more launch groups must win by more than measurement
+    noise to be chosen."""
+    best = max(res.values())
+    return min(k for k, v in res.items() if v >= best * (1 - tol))
+
+
 def tune_comb(tuned: dict, n: int, widths) -> None:
     from . import workload as wl
     b = wl.generate("schnorr", n)
@@ -92,7 +100,7 @@ def tune_comb(tuned: dict, n: int, widths) -> None:
         for x in d:
             x.free()
         gpu.close()
-    tuned["gcomb_w"] = max(res, key=res.get)
+    tuned["gcomb_w"] = _smallest_within(res, 0.01)
     tuned.setdefault("_measured", {})["gcomb_w"] = {str(k): round(v) for k, v in res.items()}
 
 
@@ -152,7 +160,7 @@ def tune_msm(tuned: dict, sizes) -> None:
     meas = {}
     for lg in sizes:
         n = 1 << lg
-        cands = range(max(2, lg - 9), min(16, lg - 1) + 1)
+        cands = range(max(2, lg - 9), min(18, lg - 1) + 1)
         best_c, best_ms = None, float("inf")
         for c in cands:
             gpu.msm_device(d_sc.ptr, d_pt.ptr, n, d_out.ptr, d_out.at(48), c)   # warm-up (buffers)
@@ -203,7 +211,7 @@ def tune_server(tuned: dict, ecdsa_items: int, schnorr_items: int) -> None:
             raise RuntimeError(f"serve_bench failed: {r.stderr[-1000:]}")
         res[groups] = json.loads(r.stdout.strip().splitlines()[-1])["batched"]["peak_sigs_per_s"]
         log(f"[tune] server_groups {groups}: {res[groups] / 1e6:.2f} M sigs/s saturated")
-    tuned["server_groups"] = max(res, key=res.get)
+    tuned["server_groups"] = _smallest_within(res, 0.01)
     tuned.setdefault("_measured", {})["server_groups"] = {str(k): round(v) for k, v in res.items()}
 
 

@@ -91,6 +91,10 @@ RocmBackend::~RocmBackend() {
     d_words_.release();
     d_slots_.release();
     if (verify_done_) (void)hipEventDestroy(verify_done_);
+    if (copy_stream_) (void)hipStreamSynchronize(copy_stream_);
+    for (hipEvent_t& ev : piece_ev_)
+        if (ev) (void)hipEventDestroy(ev);
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
@@ -151,6 +155,11 @@ GpuError RocmBackend::init(uint32_t device_index) {
         return fail(GpuError::Memory, "hipMalloc(work counter)", e);
     if ((e = hipEventCreateWithFlags(&verify_done_, hipEventDisableTiming)) != hipSuccess)
         return fail(GpuError::Backend, "hipEventCreate", e);
+    if ((e = hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking)) != hipSuccess)
+        return fail(GpuError::Queue, "hipStreamCreate(copy)", e);
+    for (hipEvent_t& ev : piece_ev_)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+            return fail(GpuError::Backend, "hipEventCreate", e);
     e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return fail(GpuError::Backend, "init synchronize", e);
     ready_ = true;
@@ -193,6 +202,39 @@ hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStrea
     return e;
 }
 
+hipError_t RocmBackend::upload_and_verify(const uint8_t* const* src, void* const* dst, const size_t* item_bytes,
+                                          const size_t* last_bytes, int nregions, const BatchDesc& whole, size_t ms,
+                                          size_t ps, size_t ss, bool collect) {
+    const size_t count = whole.count;
+    const int kv = whole.kind == kKindEcdsa ? 0 : whole.kind == kKindSchnorr ? 1 : 2;
+    const size_t round = (size_t)verify_grid_[kv] * kVerifyBlock;
+    size_t a = 0, len = round;
+    hipError_t e = hipSuccess;
+    for (int p = 0; a < count && e == hipSuccess; ++p) {
+        // a piece shorter than 2 rounds before the end takes the rest
+        size_t b = a + len;
+        if (p == kMaxPieces - 1 || b + round > count) b = count;
+        for (int k = 0; k < nregions && e == hipSuccess; ++k) {
+            const size_t bytes = (b - a) * item_bytes[k] - (b == count ? item_bytes[k] - last_bytes[k] : 0);
+            e = hipMemcpyAsync((uint8_t*)dst[k] + a * item_bytes[k], src[k] + a * item_bytes[k], bytes,
+                               hipMemcpyHostToDevice, copy_stream_);
+        }
+        if (e == hipSuccess) e = hipEventRecord(piece_ev_[p], copy_stream_);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream_, piece_ev_[p], 0);
+        if (e != hipSuccess) break;
+        BatchDesc d = whole;
+        d.count = (uint32_t)(b - a);
+        d.msgs += a * ms;
+        d.pubs += a * ps;
+        d.sigs += a * ss;
+        d.out += a;
+        e = launch_single(d, collect, stream_);
+        a = b;
+        len *= 2;
+    }
+    return e;
+}
+
 GpuError RocmBackend::verify_host(uint32_t kind, const uint8_t* msgs, const uint8_t* pubs, const uint8_t* sigs,
                                   size_t count, uint8_t* out, bool collect) {
     if (!ready_) return fail(GpuError::Device, "context not initialised");
@@ -203,18 +245,18 @@ GpuError RocmBackend::verify_host(uint32_t kind, const uint8_t* msgs, const uint
     if ((e = d_in_[0].ensure(sz_m)) != hipSuccess || (e = d_in_[1].ensure(sz_p)) != hipSuccess ||
         (e = d_in_[2].ensure(sz_s)) != hipSuccess || (e = d_out_.ensure(count)) != hipSuccess)
         return fail(GpuError::Memory, "hipMalloc(batch buffers)", e);
-    e = hipMemcpyAsync(d_in_[0].ptr, msgs, sz_m, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[1].ptr, pubs, sz_p, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_in_[2].ptr, sigs, sz_s, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess && collect)
-        e = hipMemcpyAsync(d_out_.ptr, out, count, hipMemcpyHostToDevice, stream_);
-    if (e != hipSuccess) return fail(GpuError::Memory, "copy inputs to device", e);
+    if (collect && (e = hipMemcpyAsync(d_out_.ptr, out, count, hipMemcpyHostToDevice, stream_)) != hipSuccess)
+        return fail(GpuError::Memory, "copy inputs to device", e);
     const uint8_t* dm = (const uint8_t*)d_in_[0].ptr;
     const uint8_t* dp = (const uint8_t*)d_in_[1].ptr;
     const uint8_t* ds = (const uint8_t*)d_in_[2].ptr;
     uint8_t* dout = (uint8_t*)d_out_.ptr;
-    e = launch_single(BatchDesc{kind, (uint32_t)count, dm, dp, ds, dout, 0, 0}, collect, stream_);
-    if (e != hipSuccess) return fail(GpuError::Launch, "verify kernel launch", e);
+    const uint8_t* src[3] = {msgs, pubs, sigs};
+    void* dst[3] = {d_in_[0].ptr, d_in_[1].ptr, d_in_[2].ptr};
+    const size_t item[3] = {32, pub_stride, 64};
+    e = upload_and_verify(src, dst, item, item, 3, BatchDesc{kind, (uint32_t)count, dm, dp, ds, dout, 0, 0}, 32,
+                          pub_stride, 64, collect);
+    if (e != hipSuccess) return fail(GpuError::Launch, "verify upload / launch", e);
     e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
     if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
     e = hipStreamSynchronize(stream_);
@@ -263,15 +305,20 @@ GpuError RocmBackend::ecdsa_verify_opaque_rows(const uint8_t* rows, size_t strid
     if ((e = d_in_[0].ensure(bytes)) != hipSuccess || (e = d_out_.ensure(count)) != hipSuccess)
         return fail(GpuError::Memory, "hipMalloc(row buffers)", e);
     uint8_t* d = (uint8_t*)d_in_[0].ptr;
-    if (dense)
-        e = hipMemcpyAsync(d, rows, bytes, hipMemcpyHostToDevice, stream_);
-    else
-        e = hipMemcpy2DAsync(d, kRow, rows, stride, kRow, count, hipMemcpyHostToDevice, stream_);
-    if (e != hipSuccess) return fail(GpuError::Memory, "copy rows to device", e);
     uint8_t* dout = (uint8_t*)d_out_.ptr;
-    e = launch_single(BatchDesc{kKindEcdsaOpaque, (uint32_t)count, d, d + 32, d + 65, dout, (uint32_t)pitch, 0},
-                      false, stream_);
-    if (e != hipSuccess) return fail(GpuError::Launch, "opaque-row verify launch", e);
+    const BatchDesc whole{kKindEcdsaOpaque, (uint32_t)count, d, d + 32, d + 65, dout, (uint32_t)pitch, 0};
+    if (dense) {
+        const uint8_t* src[1] = {rows};
+        void* dst[1] = {d};
+        const size_t item[1] = {stride}, last[1] = {kRow};
+        e = upload_and_verify(src, dst, item, last, 1, whole, stride, stride, stride, false);
+        if (e != hipSuccess) return fail(GpuError::Launch, "opaque-row upload / launch", e);
+    } else {
+        e = hipMemcpy2DAsync(d, kRow, rows, stride, kRow, count, hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) return fail(GpuError::Memory, "copy rows to device", e);
+        e = launch_single(whole, false, stream_);
+        if (e != hipSuccess) return fail(GpuError::Launch, "opaque-row verify launch", e);
+    }
     e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, stream_);
     if (e != hipSuccess) return fail(GpuError::Memory, "copy verdicts to host", e);
     e = hipStreamSynchronize(stream_);

@@ -132,6 +132,16 @@ private:
     template <class Launch>
     GpuError keyop_host(const char* what, size_t n, uint8_t* out, const uint8_t* const* in, const size_t* bytes,
                         int nin, Launch&& launch);
+    // Host-to-device upload of a batch pipelined with its verification: the
+    // batch is cut into pieces of r, 2r, 4r, ... items (r = lanes of the
+    // resident grid, so every launch but the last runs whole rounds), each
+    // piece is uploaded on the copy stream and verified on the context stream
+    // as soon as its inputs are in place.  Region k is item-strided
+    // (`item_bytes[k]`; the batch's last item has `last_bytes[k]`); `whole`
+    // describes the full batch in device memory.
+    hipError_t upload_and_verify(const uint8_t* const* src, void* const* dst, const size_t* item_bytes,
+                                 const size_t* last_bytes, int nregions, const BatchDesc& whole, size_t ms,
+                                 size_t ps, size_t ss, bool collect);
     // One batch through the persistent kernel (single mode).
     hipError_t launch_single(const BatchDesc& d, bool collect, hipStream_t st);
     // Verify launches share the lane slots and the work counter: a launch on
@@ -142,6 +152,9 @@ private:
     int device_ = -1;
     bool ready_ = false;
     hipStream_t stream_ = nullptr;
+    hipStream_t copy_stream_ = nullptr;        // uploads of pipelined host calls
+    static constexpr int kMaxPieces = 12;
+    hipEvent_t piece_ev_[kMaxPieces] = {};
     void* d_gcomb_ = nullptr;
     uint32_t* d_mid_ = nullptr;
     GcombDesc gdesc_{nullptr, 0, 0};

@@ -126,3 +126,34 @@ def test_abi_conventions(gpu):
     assert lib.ufsecp_gpu_is_ready(ctx) == 1
     assert lib.ufsecp_gpu_backend_name(4) == b"ROCm"
     assert lib.ufsecp_gpu_device_count(4) >= 1
+
+
+@pytest.mark.parametrize("kind,n", [("ecdsa", 600_000), ("schnorr", 450_017)])
+def test_pipelined_host_batches(gpu, kind, n):
+    """Host-buffer calls large enough to be cut into several upload pieces
+    (the resident grid holds about 2^17.6 lanes), ending with an uneven
+    remainder: verdicts equal the workload's known validity, and the collect
+    form zeroes exactly the valid items' bytes."""
+    b = wl.generate(kind, n, (wl.SEED_ECDSA if kind == "ecdsa" else wl.SEED_SCHNORR) + 5)
+    fn = gpu.ecdsa_verify_batch if kind == "ecdsa" else gpu.schnorr_verify_batch
+    v = fn(b.msgs, b.pubkeys, b.sigs)
+    assert np.array_equal(v, 1 - b.corrupted)
+    key = np.full(n, 0xA5, np.uint8)
+    lib_fn = gpu.lib.ufsecp_gpu_ecdsa_verify_collect if kind == "ecdsa" else gpu.lib.ufsecp_gpu_schnorr_verify_collect
+    assert lib_fn(gpu.ctx, b.msgs.ctypes.data, b.pubkeys.ctypes.data, b.sigs.ctypes.data, n, key.ctypes.data) == UFSECP_OK
+    assert np.array_equal(key == 0, b.corrupted == 0)
+    assert np.all(key[b.corrupted == 1] == 0xA5)
+
+
+def test_pipelined_opaque_rows(gpu):
+    """Opaque rows at a size cut into upload pieces, the buffer ending at the
+    last row's 129th byte."""
+    n, stride = 420_001, 136
+    b = wl.generate("ecdsa", n, wl.SEED_ECDSA + 6)
+    rows = np.zeros((n, stride), np.uint8)
+    rows[:, :32], rows[:, 32:65] = b.msgs, b.pubkeys
+    rows[:, 65:97] = b.sigs[:, 31::-1]
+    rows[:, 97:129] = b.sigs[:, 63:31:-1]
+    flat = rows.reshape(-1)[:(n - 1) * stride + 129].copy()
+    v = gpu.ecdsa_verify_opaque_rows(flat, stride, n)
+    assert np.array_equal(v, 1 - b.corrupted)

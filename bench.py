@@ -188,7 +188,9 @@ def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) 
 def bench_lbtc(gpu, ecdsa_batch, schnorr_batch, reps: int) -> dict:
     """libbitcoin entry points end to end from host buffers (upload, kernel,
     download, as the reference bridge measures them): ECDSA opaque rows of
-    stride 129 and BIP-340 columns, best of `reps` wall-clock calls."""
+    stride 129, BIP-340 columns, and the BIP-340 batch check (random linear
+    combination, one MSM) over the valid items; best of `reps` wall-clock
+    calls."""
     import numpy as np
     out = {"reference_published": {"ecdsa_rows_e2e_sigs_per_s": 3.56e6, "schnorr_columns_e2e_sigs_per_s": 4.53e6,
                                    "device": "RTX 5060 Ti",
@@ -219,6 +221,20 @@ def bench_lbtc(gpu, ecdsa_batch, schnorr_batch, reps: int) -> dict:
         out["schnorr_columns_n"] = b.n
         out["schnorr_columns_e2e_ms"] = round(best * 1e3, 3)
         out["schnorr_columns_e2e_sigs_per_s"] = b.n / best
+        # the same valid signatures through the random-linear-combination check
+        keep = b.corrupted == 0
+        m, p, s = (np.ascontiguousarray(a[keep]) for a in (b.msgs, b.pubkeys, b.sigs))
+        best = float("inf")
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            ok = gpu.schnorr_batch_check(m, p, s)
+            best = min(best, time.perf_counter() - t0)
+        bad = s.copy()
+        bad[len(bad) // 2, 40] ^= 1
+        parity &= bool(ok) and not gpu.schnorr_batch_check(m, p, bad)
+        out["schnorr_batch_check_n"] = int(len(m))
+        out["schnorr_batch_check_e2e_ms"] = round(best * 1e3, 3)
+        out["schnorr_batch_check_e2e_sigs_per_s"] = len(m) / best
     out["parity"] = parity
     log(f"[bench] lbtc end to end: {out}")
     return out

@@ -11,14 +11,20 @@ winners to a tuning file that every context reads at creation
   grid, by throughput of the uneven multi-batch mix (10,000-batch plan,
   first batches) and of one 2^20 batch;
 * ``msm_window <log2 n> <c>``: the Pippenger window for each size band;
-* ``server_groups``: launch groups in flight in the serving loop, by
-  saturated throughput of log-uniform requests.
+* ``server_groups``: launch groups in flight in the serving loop: among the
+  settings within 2% of the best saturated throughput of log-uniform
+  requests, the one with the lowest p99 latency at 80% of that best rate
+  (more groups in flight make each launch group smaller at a given load).
 
     python -m ultrafastsecp256k1_amd.autotune [--out PATH] [--quick]
+                                             [--only comb,grid,msm,server]
+
+With ``--only`` the other parameters are kept from the existing file.
 """
 from __future__ import annotations
 
 import argparse
+import ast
 import math
 import os
 import sys
@@ -81,8 +87,7 @@ def _time_verify(gpu, d, n, kind, reps=3) -> float:
 
 def _smallest_within(res: dict, tol: float):
     """The smallest setting whose throughput is within `tol` of the best: a
-    wider comb table or more launch groups must win by more than measurement
-    noise to be chosen."""
+    wider comb table must win by more than measurement noise to be chosen."""
     best = max(res.values())
     return min(k for k, v in res.items() if v >= best * (1 - tol))
 
@@ -192,27 +197,60 @@ def tune_msm(tuned: dict, sizes) -> None:
     gpu.close()
 
 
-def tune_server(tuned: dict, ecdsa_items: int, schnorr_items: int) -> None:
-    """Saturated throughput of the serving loop (serve_bench --peak-only,
-    log-uniform requests, zero-copy submission from a C++ client)."""
+def _serve_bench(tuned: dict, groups: int, ecdsa_items: int, schnorr_items: int, extra) -> dict:
     import json
     import subprocess
-    res = {}
-    for groups in (1, 2, 3, 4):
-        fd, path = tempfile.mkstemp(suffix=".txt", prefix="ufsecp_tune_")
-        os.close(fd)
-        write({k: v for k, v in tuned.items() if k != "_measured"}, Path(path), "candidate")
-        env = dict(os.environ, UFSECP_TUNING_FILE=path)
-        r = subprocess.run([str(PKG / "serve_bench"), "--peak-only", "--groups", str(groups), "--ecdsa-items",
-                            str(ecdsa_items), "--schnorr-items", str(schnorr_items)],
+    fd, path = tempfile.mkstemp(suffix=".txt", prefix="ufsecp_tune_")
+    os.close(fd)
+    write({k: v for k, v in tuned.items() if k != "_measured"}, Path(path), "candidate")
+    env = dict(os.environ, UFSECP_TUNING_FILE=path)
+    try:
+        r = subprocess.run([str(PKG / "serve_bench"), "--groups", str(groups), "--ecdsa-items", str(ecdsa_items),
+                            "--schnorr-items", str(schnorr_items), *extra],
                            capture_output=True, text=True, timeout=600, env=env)
+    finally:
         os.unlink(path)
-        if r.returncode != 0:
-            raise RuntimeError(f"serve_bench failed: {r.stderr[-1000:]}")
-        res[groups] = json.loads(r.stdout.strip().splitlines()[-1])["batched"]["peak_sigs_per_s"]
-        log(f"[tune] server_groups {groups}: {res[groups] / 1e6:.2f} M sigs/s saturated")
-    tuned["server_groups"] = _smallest_within(res, 0.01)
-    tuned.setdefault("_measured", {})["server_groups"] = {str(k): round(v) for k, v in res.items()}
+    if r.returncode != 0:
+        raise RuntimeError(f"serve_bench failed: {r.stderr[-1000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])["batched"]
+
+
+def tune_server(tuned: dict, ecdsa_items: int, schnorr_items: int) -> None:
+    """Serving loop (serve_bench: log-uniform requests, zero-copy submission
+    from a C++ client): saturated throughput for 1..4 groups in flight, then
+    p99 latency at 80% of the best of those rates for the settings within 2%
+    of it; the lowest p99 wins."""
+    peak = {}
+    for groups in (1, 2, 3, 4):
+        peak[groups] = _serve_bench(tuned, groups, ecdsa_items, schnorr_items, ["--peak-only"])["peak_sigs_per_s"]
+        log(f"[tune] server_groups {groups}: {peak[groups] / 1e6:.2f} M sigs/s saturated")
+    best = max(peak.values())
+    p99 = {}
+    for groups in sorted(g for g, v in peak.items() if v >= best * 0.98):
+        run = _serve_bench(tuned, groups, ecdsa_items, schnorr_items,
+                           ["--no-plain", "--loads", "0.8", "--seconds", "2", "--load-rate", str(best)])
+        p99[groups] = run["loads"][0]["p99_ms"]
+        log(f"[tune] server_groups {groups}: p99 {p99[groups]:.2f} ms at {0.8 * best / 1e6:.1f} M sigs/s")
+    tuned["server_groups"] = min(p99, key=lambda g: (p99[g], g))
+    tuned.setdefault("_measured", {})["server_groups"] = {str(k): round(v) for k, v in peak.items()}
+    tuned["_measured"]["server_p99_ms_at_80pct"] = {str(k): v for k, v in p99.items()}
+
+
+def read(path: Path) -> dict:
+    """The parameters (and the measured record) of an existing tuning file."""
+    tuned: dict = {}
+    for line in path.read_text().splitlines():
+        if line.startswith("# measured: "):
+            tuned["_measured"] = ast.literal_eval(line[len("# measured: "):])
+            continue
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "msm_window":
+            tuned.setdefault("msm_window", {})[int(f[1])] = int(f[2])
+        else:
+            tuned[f[0]] = int(f[1])
+    return tuned
 
 
 def write(tuned: dict, path: Path, device: str) -> None:
@@ -232,18 +270,28 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(DEFAULT_OUT))
     ap.add_argument("--quick", action="store_true", help="smaller measurement sizes")
+    ap.add_argument("--only", default="comb,grid,msm,server",
+                    help="comma list of comb, grid, msm, server; the rest is kept from --out")
     args = ap.parse_args()
+    only = set(args.only.split(","))
+    if not only <= {"comb", "grid", "msm", "server"}:
+        ap.error(f"unknown --only item in {args.only}")
     from . import UfsecpGpu
     g = UfsecpGpu()
     dev = g.lib.ufsecp_gpu_backend_name(4).decode()
     g.close()
-    tuned: dict = {}
+    out = Path(args.out)
+    tuned: dict = read(out) if only != {"comb", "grid", "msm", "server"} and out.exists() else {}
     n = 1 << (17 if args.quick else 20)
     t0 = time.perf_counter()
-    tune_comb(tuned, n, (12, 14, 16, 17, 18))
-    tune_grid(tuned, n, 500 if args.quick else 2000)
-    tune_msm(tuned, (8, 10, 12, 14, 16) if args.quick else (8, 10, 12, 14, 16, 18, 20))
-    tune_server(tuned, *((1 << 19, 1 << 18) if args.quick else (1 << 21, 1 << 20)))
+    if "comb" in only:
+        tune_comb(tuned, n, (12, 14, 16, 17, 18))
+    if "grid" in only:
+        tune_grid(tuned, n, 500 if args.quick else 2000)
+    if "msm" in only:
+        tune_msm(tuned, (8, 10, 12, 14, 16) if args.quick else (8, 10, 12, 14, 16, 18, 20))
+    if "server" in only:
+        tune_server(tuned, *((1 << 19, 1 << 18) if args.quick else (1 << 21, 1 << 20)))
     write(tuned, Path(args.out), dev)
     log(f"[tune] wrote {args.out} in {time.perf_counter() - t0:.1f} s")
     print(Path(args.out).read_text())

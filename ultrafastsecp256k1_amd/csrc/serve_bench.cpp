@@ -9,11 +9,14 @@
 // has something to keep) and 1% of the items are corrupted.  For each server
 // the sustained peak is measured first (a saturating backlog); then arrivals at
 // 50%, 80% and 95% of that peak run for a fixed time and the latency of every
-// request (arrival -> verdicts in the caller's buffer) is recorded.
+// request (arrival -> verdicts in the caller's buffer) is recorded.  With
+// --load-rate R the batched loads are fractions of R sigs/s instead of the
+// measured peak (the same offered load for every server configuration).
 //
 //   serve_bench [--seconds S] [--loads 0.5,0.8,0.95] [--ecdsa-items N]
 //               [--schnorr-items N] [--max-size N] [--cache-log2 K]
 //               [--groups G] [--budget-mb M] [--no-plain] [--peak-only]
+//               [--load-rate R]
 // Prints one JSON object.
 #include <algorithm>
 #include <atomic>
@@ -82,6 +85,7 @@ struct Args {
     uint64_t budget_mb = 512;
     bool plain = true;
     bool peak_only = false;
+    double load_rate = 0;   // > 0: batched loads are fractions of this rate
 };
 
 [[noreturn]] void die(const char* what, int rc) {
@@ -346,6 +350,7 @@ int main(int argc, char** argv) {
         else if (s == "--groups") a.groups = (uint32_t)std::atoi(val());
         else if (s == "--budget-mb") a.budget_mb = std::strtoull(val(), nullptr, 10);
         else if (s == "--no-plain") a.plain = false;
+        else if (s == "--load-rate") a.load_rate = std::atof(val());
         else if (s == "--peak-only") {
             a.peak_only = true;
             a.plain = false;
@@ -411,8 +416,9 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "[serve] batched peak %.2f M sigs/s (%llu requests)\n", peak.achieved / 1e6,
                  (unsigned long long)peak.requests);
     std::vector<RunResult> runs;
+    const double load_base = a.load_rate > 0 ? a.load_rate : peak.achieved;
     for (double f : a.loads) {
-        RunResult r = run_batched(srv, pools, plan, f * peak.achieved / msize, a.seconds, outbuf);
+        RunResult r = run_batched(srv, pools, plan, f * load_base / msize, a.seconds, outbuf);
         std::fprintf(stderr, "[serve] batched %.0f%%: %.2f M sigs/s p50 %.2f ms p99 %.2f ms (%llu req, %llu bad)\n",
                      f * 100, r.achieved / 1e6, r.p50, r.p99, (unsigned long long)r.requests,
                      (unsigned long long)r.mismatches);

@@ -168,3 +168,69 @@ def test_field_mul_sqr_chains(gpu):
     out = gpu.selftest(7, a, b)
     for i, x in enumerate(xs):
         assert from_limbs(out[i, :8]) == pow(x, 1 << 4096, P)
+
+
+# ---------------------------------------------------------------------------
+# 10 x 26-bit limb layer (secp_field26.h) used by the scalar-multiplication
+# loops: products at every input magnitude the multiply allows (up to 20 x
+# the limb maximum), the documented output bounds, weak normalisation and the
+# 8 x 32 round trip.
+# ---------------------------------------------------------------------------
+OP_FE26_MUL, OP_FE26_SQR, OP_FE26_NORM, OP_FE26_FROMTO = 30, 31, 32, 33
+M26, M22 = (1 << 26) - 1, (1 << 22) - 1
+LIM26 = [M26] * 9 + [M22]
+
+
+def _v26(limbs):
+    return sum(int(x) << (26 * i) for i, x in enumerate(limbs))
+
+
+def _limbs26(rnd, mag):
+    return [rnd.randint(0, mag * lim) for lim in LIM26]
+
+
+def _pack26(rows):
+    a = np.zeros((len(rows), 16), np.uint32)
+    for i, r in enumerate(rows):
+        a[i, :10] = r
+    return a
+
+
+def test_fe26_mul_sqr(gpu):
+    rnd = random.Random(26)
+    xs, ys = [], []
+    for mag in (1, 2, 4, 8, 12, 16, 20):
+        xs.append([mag * lim for lim in LIM26])        # every limb at its maximum
+        ys.append([mag * lim for lim in LIM26])
+        for _ in range(40):
+            xs.append(_limbs26(rnd, mag))
+            ys.append(_limbs26(rnd, rnd.choice((1, 2, mag))))
+    xs += [[0] * 10, [1] + [0] * 9, [((P >> (26 * i)) & lim) for i, lim in enumerate(LIM26)]]
+    ys += [[M26] * 9 + [M22], [0] * 10, [2] + [0] * 9]
+    a, b = _pack26(xs), _pack26(ys)
+    for op, f in ((OP_FE26_MUL, lambda x, y: x * y), (OP_FE26_SQR, lambda x, y: x * x)):
+        out = gpu.selftest(op, a, b)
+        for i, (x, y) in enumerate(zip(xs, ys)):
+            want = f(_v26(x), _v26(y)) % P
+            assert from_limbs(out[i, :8]) % P == want, (op, i)
+            r26 = out[i, 8:18]
+            assert all(int(r26[l]) <= M26 for l in (0, 1, 3, 4, 5, 6, 7, 8)), (op, i)
+            assert int(r26[2]) <= M26 + (1 << 22) and int(r26[9]) <= M22, (op, i)
+            assert _v26(r26) % P == want
+
+
+def test_fe26_norm_and_round_trip(gpu):
+    rnd = random.Random(27)
+    rows = [[(1 << 31) - 1] * 10, [0] * 10, [((P >> (26 * i)) & lim) for i, lim in enumerate(LIM26)]]
+    rows += [[rnd.randrange(1 << 31) for _ in range(10)] for _ in range(300)]
+    rows += [_limbs26(rnd, m) for m in (1, 2, 24) for _ in range(50)]
+    out = gpu.selftest(OP_FE26_NORM, _pack26(rows), _pack26(rows))
+    for i, r in enumerate(rows):
+        n = [int(v) for v in out[i, :10]]
+        assert all(v <= M26 for v in n[:9]) and n[9] <= (1 << 22) + 64, i
+        assert _v26(n) < 2 * P and _v26(n) % P == _v26(r) % P, i
+    xs = _vals(200, 28)
+    out = gpu.selftest(OP_FE26_FROMTO, _pack(xs), _pack(xs))
+    for i, x in enumerate(xs):
+        assert _v26(out[i, 8:18]) == x, i                 # exact limbs
+        assert from_limbs(out[i, :8]) % P == x % P and from_limbs(out[i, :8]) < 2**256, i

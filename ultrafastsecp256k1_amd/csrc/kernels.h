@@ -37,6 +37,12 @@ enum SelftestOp : int {
     kOpScInvFermat = 13,
     kOpGenMul = 20,
     kOpVarMul = 21,
+    // 10 x 26-bit limbs: inputs are 10 limbs (any values the op allows),
+    // outputs the 8 x 32 result of fe26_to_fe, or the 10 limbs of fe26_norm.
+    kOpFe26Mul = 30,
+    kOpFe26Sqr = 31,
+    kOpFe26Norm = 32,
+    kOpFe26FromTo = 33,
 };
 
 // Signature kinds of a verify batch.

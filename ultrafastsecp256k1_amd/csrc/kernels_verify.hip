@@ -382,6 +382,31 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         o[16] = R.inf; o[17] = ok;
         if (!R.inf) { aff A; jac_to_aff(A, R); for (int l = 0; l < 8; ++l) { o[l] = A.x.v[l]; o[8 + l] = A.y.v[l]; } }
     } break;
+    case kOpFe26Mul:
+    case kOpFe26Sqr:
+    case kOpFe26Norm: {
+        fe26 x26, y26, r26;
+        for (int l = 0; l < 10; ++l) {
+            x26.v[l] = a[16 * i + l];
+            y26.v[l] = b[16 * i + l];
+        }
+        if (op == kOpFe26Norm) {
+            fe26_norm(r26, x26);
+            for (int l = 0; l < 10; ++l) o[l] = r26.v[l];
+            break;
+        }
+        if (op == kOpFe26Mul) fe26_mul(r26, x26, y26); else fe26_sqr(r26, x26);
+        for (int l = 0; l < 10; ++l) o[8 + l] = r26.v[l];
+        fe26_to_fe(r, r26);
+        for (int l = 0; l < 8; ++l) o[l] = r.v[l];
+    } break;
+    case kOpFe26FromTo: {   // 8 x 32 -> 10 x 26 -> 8 x 32
+        fe26 x26;
+        fe26_from_fe(x26, x);
+        fe26_to_fe(r, x26);
+        for (int l = 0; l < 8; ++l) o[l] = r.v[l];
+        for (int l = 0; l < 10; ++l) o[8 + l] = x26.v[l];
+    } break;
     default: break;
     }
 }

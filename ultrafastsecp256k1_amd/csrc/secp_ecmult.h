@@ -18,7 +18,7 @@
 // G tables like here.  The verdict depends only on the exact group element, so
 // any exact algorithm yields identical results.
 #pragma once
-#include "secp_point.h"
+#include "secp_point26.h"
 
 namespace ufs {
 
@@ -283,14 +283,15 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
     constexpr int kVarSteps = kPWindows * (kPW + 2);
     const int nsteps = kVarSteps + 3 + 1 + G.nwin;
 
-    // accumulator starts at the top digits, P' + phi(P') (both 1: k < 2^128):
-    // P' itself here, phi(P') as the first scheduled addition below.
-    jac acc;
+    // accumulator (10 x 26-bit limbs) starts at the top digits,
+    // P' + phi(P') (both 1: k < 2^128): P' itself here, phi(P') as the first
+    // scheduled addition below.
+    jac26 acc;
     {
         aff e;
         load_aff(e, slot);
         fe_cond_neg(e.y, neg1);
-        jac_from_aff(acc, e);
+        jac26_from_aff(acc, e);
     }
     bool pending = false;
     int step = -1;   // step -1 adds phi(P') for the top digit of k2
@@ -303,8 +304,8 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
             is_dbl = r < kPW;
         }
         if (is_dbl) {
-            jac t;
-            jac_dbl_body<true>(t, acc);
+            jac26 t;
+            jac26_dbl(t, acc);
             if (fixup) {
                 if (pending) acc = t;
                 pending = false;
@@ -314,7 +315,9 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
         } else if (step == kVarSteps + 2) {
             fe zs;
             load_fe(zs, &slot[8].x);
-            fe_mul(acc.z, acc.z, zs);
+            fe26 zs26;
+            fe26_from_fe(zs26, zs);
+            fe26_mul(acc.z, acc.z, zs26);
         } else {
             const aff* bp;
             u32 fl;
@@ -350,9 +353,9 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
                 bp = G.tab + (size_t)i * half + idx;
                 fl = ng ? (fg ^ kAddNeg) : fg;
             }
-            jac t;
+            jac26 t;
             bool nd;
-            jac_madd_mem_body(t, acc, bp, fl, nd);
+            jac26_madd_mem(t, acc, bp, fl, nd);
             if (keep) {
                 acc = t;
                 pending = nd;
@@ -360,7 +363,7 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
         }
         ++step;
     }
-    out = acc;
+    jac26_to_jac(out, acc);
 }
 
 // u*G from the fixed-base comb alone (any u < n, u = 0 gives infinity): the
@@ -380,28 +383,28 @@ UFS_DEV void ecmult_gen(jac& out, const gcomb& G, const sc& u) {
     }
     const u32 fg = ueven ? kAddNeg : 0u;
     const int half = 1 << (G.w - 1);
-    jac acc;
+    jac26 acc;
     {
         aff e;
         load_aff(e, G.tab + (size_t)G.nwin * half);
         fe_cond_neg(e.y, ueven);
-        jac_from_aff(acc, e);
+        jac26_from_aff(acc, e);
     }
     for (int i = 0; i < G.nwin; ++i) {
         u32 idx;
         bool ng;
         odd_digit(ku.v, i, G.w, idx, ng);
-        jac t;
+        jac26 t;
         bool nd;
-        jac_madd_mem_body(t, acc, G.tab + (size_t)i * half + idx, ng ? (fg ^ kAddNeg) : fg, nd);
+        jac26_madd_mem(t, acc, G.tab + (size_t)i * half + idx, ng ? (fg ^ kAddNeg) : fg, nd);
         if (__any(nd)) {
-            jac d;
-            jac_dbl(d, acc);
+            jac26 d;
+            jac26_dbl(d, acc);
             if (nd) t = d;
         }
         acc = t;
     }
-    out = acc;
+    jac26_to_jac(out, acc);
 }
 
 }  // namespace ufs

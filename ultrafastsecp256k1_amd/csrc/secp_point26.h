@@ -1,0 +1,154 @@
+// Jacobian doubling and mixed addition on 10 x 26-bit limbs, for the
+// accumulators of the scalar-multiplication loops (secp_ecmult.h).  Same
+// formulas and the same exception handling as the 8 x 32 versions in
+// secp_point.h (dbl-2009-l, madd-2007-bl, a = 0, so they also run on the
+// isomorphic curves y^2 = x^3 + 7 Z^6 of the point tables); the addend is
+// read in the 8 x 32 storage form and converted.
+//
+// Lazy reduction: the constants K of fe26_sub<K> and the two weak
+// normalisations (D in the doubling; h and r in the addition) come from
+// worst-case limb-bound propagation through both formulas, iterated to a fixed
+// point (X, Y, Z of magnitude at most 6.1, 10.1, 5.1 on entry and exit, every
+// multiplication input at most 16).  Every accumulator value the loops feed
+// in is within those bounds: affine entries (exact 26-bit limbs, y negated as
+// 2p - y), multiplication outputs, and the formulas' own outputs.
+#pragma once
+#include "secp_field26.h"
+#include "secp_point.h"
+
+namespace ufs {
+
+struct jac26 { fe26 x, y, z; u32 inf; };
+
+UFS_DEV void fe26_set_u32(fe26& r, u32 x) {
+    r.v[0] = x;
+#pragma unroll
+    for (int i = 1; i < 10; ++i) r.v[i] = 0;
+}
+
+UFS_DEV void jac26_set_inf(jac26& r) {
+    fe26_set_u32(r.x, 1);
+    fe26_set_u32(r.y, 1);
+    fe26_set_u32(r.z, 0);
+    r.inf = 1;
+}
+
+UFS_DEV void jac26_from_aff(jac26& r, const aff& a) {
+    fe26_from_fe(r.x, a.x);
+    fe26_from_fe(r.y, a.y);
+    fe26_set_u32(r.z, 1);
+    r.inf = 0;
+}
+
+UFS_DEV void jac26_to_jac(jac& r, const jac26& a) {
+    fe26_to_fe(r.x, a.x);
+    fe26_to_fe(r.y, a.y);
+    fe26_to_fe(r.z, a.z);
+    r.inf = a.inf;
+}
+
+// r = 2a (dbl-2009-l: 2M + 5S).  2a = inf iff a = inf.
+UFS_DEV void jac26_dbl(jac26& r, const jac26& a) {
+    fe26 A, B, C, D, E, F, t;
+    fe26_sqr(A, a.x);          // A = X^2
+    fe26_sqr(B, a.y);          // B = Y^2
+    fe26_sqr(C, B);            // C = B^2
+    fe26_add(t, a.x, B);
+    fe26_sqr(t, t);
+    fe26_sub<2>(t, t, A);
+    fe26_sub<2>(t, t, C);
+    fe26_add(t, t, t);
+    fe26_norm(D, t);           // D = 2((X+B)^2 - A - C)
+    fe26_mul3(E, A);           // E = 3A
+    fe26_sqr(F, E);            // F = E^2
+    fe26 z3;
+    fe26_mul(z3, a.y, a.z);
+    fe26_add(r.z, z3, z3);     // Z3 = 2YZ
+    fe26_add(t, D, D);
+    fe26_sub<3>(r.x, F, t);    // X3 = F - 2D
+    fe26_sub<5>(t, D, r.x);
+    fe26_mul(t, E, t);
+    fe26_shl<3>(C, C);
+    fe26_sub<9>(r.y, t, C);    // Y3 = E(D - X3) - 8C
+    r.inf = a.inf;
+}
+
+// out = a + b for b affine (8 x 32, never infinity), flags kAddNeg / kAddBeta
+// as in secp_point.h.  a == b (finite) leaves out = a and sets `need_dbl`
+// (the caller doubles); a == -b gives infinity; a = inf gives b.
+UFS_DEV void jac26_madd(jac26& out, const jac26& a, const aff& b8, u32 flags, bool& need_dbl) {
+    fe26 bx, by;
+    fe26_from_fe(bx, b8.x);
+    fe26_from_fe(by, b8.y);
+    if (flags & kAddBeta) {
+        const fe BETA = {{UFS_BETA_LIMBS}};
+        fe26 beta;
+        fe26_from_fe(beta, BETA);
+        fe26_mul(bx, bx, beta);
+    }
+    {
+        fe26 ny;
+        fe26_neg<2>(ny, by);
+        const bool ng = (flags & kAddNeg) != 0;
+#pragma unroll
+        for (int l = 0; l < 10; ++l) by.v[l] = ng ? ny.v[l] : by.v[l];
+    }
+    fe26 z1z1, u2, s2, h, hh, i4, j, rr, v, t;
+    fe26_sqr(z1z1, a.z);
+    fe26_mul(u2, bx, z1z1);
+    fe26_mul(s2, a.z, z1z1);
+    fe26_mul(s2, by, s2);
+    fe26_sub<7>(t, u2, a.x);
+    fe26_norm(h, t);           // H = U2 - X1
+    fe26_sub<11>(t, s2, a.y);
+    fe26_add(t, t, t);
+    fe26_norm(rr, t);          // r = 2(S2 - Y1)
+    fe26_sqr(hh, h);           // HH = H^2
+    fe26_shl<2>(i4, hh);       // I = 4HH
+    fe26_mul(j, h, i4);        // J = H*I
+    fe26_mul(v, a.x, i4);      // V = X1*I
+    fe26_sqr(t, rr);
+    fe26_sub<2>(t, t, j);
+    fe26 v2;
+    fe26_add(v2, v, v);
+    fe26_sub<3>(out.x, t, v2); // X3 = r^2 - J - 2V
+    fe26_sub<7>(t, v, out.x);
+    fe26_mul(t, rr, t);
+    fe26_mul(j, a.y, j);
+    fe26_add(j, j, j);
+    fe26_sub<3>(out.y, t, j);  // Y3 = r(V - X3) - 2 Y1 J
+    fe26_add(t, a.z, h);
+    fe26_sqr(t, t);
+    fe26_sub<2>(t, t, z1z1);
+    fe26_sub<2>(out.z, t, hh); // Z3 = (Z1 + H)^2 - Z1Z1 - HH
+    out.inf = 0;
+    need_dbl = false;
+    // Rare cases (a = inf, H = 0) behind a wave-uniform test: H and r are
+    // normalised, so a zero H has limb 2 equal to 0 or 2^26 - 1.
+    if (__any(a.inf | fe26_normed_maybe_zero(h))) {
+        const bool degen = fe26_normed_is_zero(h) & !a.inf;
+        const bool same = fe26_normed_is_zero(rr);
+        if (a.inf) {
+            out.x = bx;
+            out.y = by;
+            fe26_set_u32(out.z, 1);
+            out.inf = 0;
+        }
+        need_dbl = degen & same;
+        if (degen) {
+            if (same) {
+                out = a;
+            } else {
+                jac26_set_inf(out);
+            }
+        }
+    }
+}
+
+UFS_DEV void jac26_madd_mem(jac26& out, const jac26& a, const aff* bp, u32 flags, bool& need_dbl) {
+    aff b;
+    load_aff_g(b, bp);
+    jac26_madd(out, a, b, flags, need_dbl);
+}
+
+}  // namespace ufs

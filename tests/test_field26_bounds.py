@@ -1,0 +1,198 @@
+"""Overflow bounds of the 10 x 26-bit field layer (secp_field26.h,
+secp_point26.h), checked on the CPU.
+
+The layer reduces lazily: additions and subtractions are limb-wise 32-bit adds
+and the multiply accumulates 64-bit column sums without carry-outs.  Its
+correctness therefore rests on limb bounds, which this test derives:
+
+* the multiply's worst case (every input limb at its maximum) for the largest
+  input magnitude the layer allows, replaying the exact column / fold / carry
+  sequence of fe26_columns with 64-bit and 32-bit overflow checks (every
+  intermediate is monotone in the inputs, so the all-maximum input bounds
+  every other input);
+* limb-bound propagation through the doubling and the mixed addition exactly
+  as secp_point26.h writes them, iterated from the loop's entry values to a
+  fixed point: every multiply input stays within the allowed magnitude, every
+  fe26_norm input below 2^31, every limb below 2^32, and each fe26_sub<K> has
+  K * p limb-wise at least its subtrahend's bound.  The K constants written
+  in secp_point26.h are asserted to be sufficient.
+"""
+P = 2**256 - 2**32 - 977
+M26, M22, M32, M64 = (1 << 26) - 1, (1 << 22) - 1, (1 << 32) - 1, (1 << 64) - 1
+LIM = [M26] * 9 + [M22]
+PL = [(P >> (26 * i)) & lim for i, lim in enumerate(LIM)]
+R0, R1 = 0x3D10, 0x400
+MUL_MAX_MAG = 20          # largest input magnitude the multiply accepts
+LOOP_MUL_MAG = 16         # largest magnitude the loops feed it
+
+# fe26_sub<K> / fe26_neg<K> constants as written in secp_point26.h
+DBL_K = {"-A": 2, "-C": 2, "X3": 3, "D-X3": 5, "Y3": 9}
+MADD_K = {"h": 7, "s2-y": 11, "-j": 2, "X3": 3, "v-X3": 7, "Y3": 3, "-z1z1": 2, "Z3": 2, "neg y": 2}
+
+
+def test_constants():
+    assert sum(l << (26 * i) for i, l in enumerate(PL)) == P
+    assert PL[:2] == [0x3FFFC2F, 0x3FFFFBF] and PL[2:9] == [M26] * 7 and PL[9] == M22
+    assert (2**260 - (R1 * 2**26 + R0)) % P == 0
+
+
+def _mad(a, b, c):
+    assert 0 <= a <= M32 and 0 <= b <= M32
+    r = a * b + c
+    assert r <= M64, "64-bit column overflow"
+    return r
+
+
+def mul_worst(mag):
+    """fe26_columns on all-maximum inputs of magnitude `mag`; returns the
+    output limbs and the intermediate maxima (t, h[9])."""
+    a = [mag * lim for lim in LIM]
+    b = list(a)
+    h = [0] * 10
+    acc = 0
+    for k in range(10, 19):
+        for i in range(k - 9, 10):
+            acc = _mad(a[i], b[k - i], acc)
+        h[k - 10] = acc & M26
+        acc >>= 26
+    h[9] = acc
+    assert h[9] <= M32
+    l = [0] * 10
+    acc = 0
+    for k in range(10):
+        for i in range(k + 1):
+            acc = _mad(a[i], b[k - i], acc)
+        acc = _mad(h[k], R0, acc)
+        if k >= 1:
+            acc = _mad(h[k - 1], R1, acc)
+        if k == 0:
+            acc = _mad(h[9], R1 * R0, acc)
+        if k == 1:
+            acc = _mad(h[9], R1 * R1, acc)
+        if k < 9:
+            l[k] = acc & M26
+            acc >>= 26
+        else:
+            l[9] = acc & M22
+            acc >>= 22
+    t = acc
+    assert t < 2**42
+    x0 = t * 977 + M26                      # l[0] at most M26
+    x1 = (t << 6) + (x0 >> 26) + M26
+    assert x0 <= M64 and x1 <= M64
+    return {"t": t, "h9": h[9], "l2_excess": x1 >> 26}
+
+
+def test_multiply_worst_case():
+    w = mul_worst(MUL_MAX_MAG)
+    assert w["l2_excess"] <= 1 << 22          # documented: l[2] <= 2^26 - 1 + 2^22
+    try:
+        mul_worst(24)
+        raised = False
+    except AssertionError:
+        raised = True
+    assert raised, "the multiply's limit moved: recheck MUL_MAX_MAG"
+
+
+# --------------------------------------------------------------------------
+# bound propagation through the formulas
+# --------------------------------------------------------------------------
+OUT = [M26, M26, M26 + (1 << 22)] + [M26] * 6 + [M22]     # multiply / square output
+NORM_OUT = [M26] * 9 + [(1 << 22) + 64]
+
+
+def _mag(v):
+    return max(x / l for x, l in zip(v, LIM))
+
+
+def _chk(v):
+    assert all(x <= M32 for x in v)
+    return v
+
+
+def mul(a, b):
+    assert _mag(a) <= LOOP_MUL_MAG and _mag(b) <= LOOP_MUL_MAG
+    return list(OUT)
+
+
+def add(a, b):
+    return _chk([x + y for x, y in zip(a, b)])
+
+
+def shl(a, s):
+    return _chk([x << s for x in a])
+
+
+def sub(a, b, k):
+    assert all(k * pl >= x for pl, x in zip(PL, b)), "fe26_sub K too small"
+    return _chk([x + k * pl for x, pl in zip(a, PL)])
+
+
+def norm(a):
+    assert all(x < 2**31 for x in a)
+    return list(NORM_OUT)
+
+
+def dbl(X, Y, Z):
+    A = mul(X, X)
+    B = mul(Y, Y)
+    C = mul(B, B)
+    t = mul(add(X, B), add(X, B))
+    t = sub(t, A, DBL_K["-A"])
+    t = sub(t, C, DBL_K["-C"])
+    D = norm(add(t, t))
+    E = _chk([3 * x for x in A])
+    F = mul(E, E)
+    Z3 = mul(Y, Z)
+    Z3 = add(Z3, Z3)
+    X3 = sub(F, add(D, D), DBL_K["X3"])
+    t = mul(E, sub(D, X3, DBL_K["D-X3"]))
+    Y3 = sub(t, shl(C, 3), DBL_K["Y3"])
+    return X3, Y3, Z3
+
+
+def madd(X, Y, Z, bx, by):
+    z1z1 = mul(Z, Z)
+    u2 = mul(bx, z1z1)
+    s2 = mul(by, mul(Z, z1z1))
+    h = norm(sub(u2, X, MADD_K["h"]))
+    t = sub(s2, Y, MADD_K["s2-y"])
+    rr = norm(add(t, t))
+    hh = mul(h, h)
+    i4 = shl(hh, 2)
+    j = mul(h, i4)
+    v = mul(X, i4)
+    t = sub(mul(rr, rr), j, MADD_K["-j"])
+    X3 = sub(t, add(v, v), MADD_K["X3"])
+    t = mul(rr, sub(v, X3, MADD_K["v-X3"]))
+    j = mul(Y, j)
+    Y3 = sub(t, add(j, j), MADD_K["Y3"])
+    t = mul(add(Z, h), add(Z, h))
+    t = sub(t, z1z1, MADD_K["-z1z1"])
+    Z3 = sub(t, hh, MADD_K["Z3"])
+    return X3, Y3, Z3
+
+
+def _vmax(*vs):
+    return [max(c) for c in zip(*vs)]
+
+
+def test_formulas_reach_a_fixed_point():
+    exact = list(LIM)                                   # fe26_from_fe of a stored coordinate
+    neg_y = sub([0] * 10, exact, MADD_K["neg y"])       # fe26_neg<2>
+    bx = _vmax(exact, OUT)                              # x or beta * x
+    by = _vmax(exact, neg_y)
+    one = [1] + [0] * 9
+    # entry values: an affine point (jac26_from_aff), infinity (1, 1, 0)
+    X, Y, Z = exact, exact, _vmax(one, OUT)
+    for _ in range(10):
+        d = dbl(X, Y, Z)
+        m = madd(X, Y, Z, bx, by)
+        # also the madd's own special outputs: b itself (a = inf) and a itself
+        Xn, Yn, Zn = _vmax(X, d[0], m[0], bx), _vmax(Y, d[1], m[1], by), _vmax(Z, d[2], m[2], OUT)
+        if (Xn, Yn, Zn) == (X, Y, Z):
+            break
+        X, Y, Z = Xn, Yn, Zn
+    else:
+        raise AssertionError("no fixed point")
+    assert _mag(X) < 6.2 and _mag(Y) < 10.2 and _mag(Z) < 5.2

@@ -294,8 +294,9 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* 
         const bool after = end < E && (u32)(entries[end] >> 32) == (u32)(entries[end - 1] >> 32);
         u32 cur = (u32)(ent >> 32);
         bool first = true;
-        jac acc;
-        jac_set_inf(acc);
+        // the running sum on 10 x 26-bit limbs, converted when a bucket ends
+        jac26 acc;
+        jac26_set_inf(acc);
         aff nxt = pts[(u32)ent & 0x7fffffffu];
         for (u32 e = beg; e < end; ++e) {
             const u32 k = (u32)(ent >> 32);
@@ -306,26 +307,30 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* 
                 nxt = pts[(u32)ent & 0x7fffffffu];
             }
             if (k != cur) {
-                if (first && before) { hk = cur; hv = acc; } else { bsum[cur] = acc; }
+                jac a8;
+                jac26_to_jac(a8, acc);
+                if (first && before) { hk = cur; hv = a8; } else { bsum[cur] = a8; }
                 first = false;
                 cur = k;
-                jac_set_inf(acc);
+                jac26_set_inf(acc);
             }
             // inline mixed addition; a == P (possible for repeated points)
             // doubles instead
-            jac t;
+            jac26 t;
             bool nd;
-            jac_madd_reg_body(t, acc, P, neg ? kAddNeg : 0u, nd);
+            jac26_madd(t, acc, P, neg ? kAddNeg : 0u, nd);
             if (__any(nd)) {
-                jac dd;
-                jac_dbl(dd, acc);
+                jac26 dd;
+                jac26_dbl(dd, acc);
                 if (nd) t = dd;
             }
             acc = t;
         }
-        if (first && before) { hk = cur; hv = acc; }
-        else if (after) { tk = cur; tv = acc; }
-        else { bsum[cur] = acc; }
+        jac a8;
+        jac26_to_jac(a8, acc);
+        if (first && before) { hk = cur; hv = a8; }
+        else if (after) { tk = cur; tv = a8; }
+        else { bsum[cur] = a8; }
     }
     okeys[2 * j] = hk;
     opart[2 * j] = hv;
@@ -511,10 +516,19 @@ __global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u3
         *status = (e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */;
         return;
     }
+    // One lane, so the time is the dependent-instruction latency of the
+    // (W - 1) c doublings: they run on 10 x 26-bit limbs, whose additions and
+    // subtractions are independent per limb instead of carry chains.
     jac acc;
     jac_add(acc, wsum[W - 1], wsum[W]);   // the top window's two bucket groups
     for (int w = (int)W - 2; w >= 0; --w) {
-        for (u32 d = 0; d < c; ++d) jac_dbl(acc, acc);
+        jac26 a26;
+        fe26_from_fe(a26.x, acc.x);
+        fe26_from_fe(a26.y, acc.y);
+        fe26_from_fe(a26.z, acc.z);
+        a26.inf = acc.inf;
+        for (u32 d = 0; d < c; ++d) jac26_dbl(a26, a26);
+        jac26_to_jac(acc, a26);
         jac_add(acc, acc, wsum[w]);
     }
     if (acc.inf) {

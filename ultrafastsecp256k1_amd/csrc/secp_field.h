@@ -433,18 +433,6 @@ UFS_DEV void fe_inv_fermat(fe& r, const fe& a) {
     fe_sqr_n(t, t, 2);     fe_mul(r, t, a);
 }
 
-// r = a^((p+1)/4); returns true iff r^2 == a (a is a quadratic residue).
-UFS_DEV bool fe_sqrt(fe& r, const fe& a) {
-    fe x223, x22, x2, t;
-    fe_pow_x223(x223, x22, x2, a);
-    fe_sqr_n(t, x223, 23); fe_mul(t, t, x22);
-    fe_sqr_n(t, t, 6);     fe_mul(t, t, x2);
-    fe_sqr_n(t, t, 2);
-    r = t;
-    fe_sqr(t, r);
-    return fe_equal(t, a);
-}
-
 // Big-endian 32 bytes -> raw limbs (no reduction).
 UFS_DEV void u256_from_be(u32 r[8], const u8* b) {
 #pragma unroll

@@ -234,3 +234,79 @@ def test_fe26_norm_and_round_trip(gpu):
     for i, x in enumerate(xs):
         assert _v26(out[i, 8:18]) == x, i                 # exact limbs
         assert from_limbs(out[i, :8]) % P == x % P and from_limbs(out[i, :8]) < 2**256, i
+
+
+OP_JAC26_MADD, OP_JAC26_DBL = 34, 35
+
+
+def _aff_pack(points):
+    a = np.zeros((len(points), 16), np.uint32)
+    for i, (x, y) in enumerate(points):
+        a[i, :8] = to_limbs(x)
+        a[i, 8:] = to_limbs(y)
+    return a
+
+
+def test_jac26_madd_and_dbl_exceptional_cases(gpu):
+    """The 10 x 26 mixed addition against the group law, including its
+    exceptional paths: B' = A (doubling requested, accumulator unchanged),
+    B' = -A (infinity), A at infinity (result B'), with negated and
+    beta-mapped addends; and the doubling, infinity included.  Item i uses
+    Z = i + 3, flags i & 3, A = infinity when i % 11 == 10."""
+    rnd = random.Random(34)
+    As, Bs = [], []
+    n = 220
+    for i in range(n):
+        A = mul(rnd.randrange(1, N))
+        flags = i & 3
+        kind = i % 5
+        if kind == 0:     # B' = A: choose B so that the flag transform gives A
+            x, y = A
+            if flags & 2:
+                x = x * pow(BETA, 2, P) % P      # beta^3 = 1, so beta^2 x maps to x
+            if flags & 1:
+                y = (P - y) % P
+            B = (x, y)
+        elif kind == 1:   # B' = -A
+            x, y = A
+            if flags & 2:
+                x = x * pow(BETA, 2, P) % P
+            if not flags & 1:
+                y = (P - y) % P
+            B = (x, y)
+        else:
+            B = mul(rnd.randrange(1, N))
+        As.append(A)
+        Bs.append(B)
+    a, b = _aff_pack(As), _aff_pack(Bs)
+    out = gpu.selftest(OP_JAC26_MADD, a, b)
+    for i in range(n):
+        flags = i & 3
+        bx, by = Bs[i]
+        if flags & 2:
+            bx = bx * BETA % P
+        if flags & 1:
+            by = (P - by) % P
+        Bp = (bx, by)
+        a_inf = i % 11 == 10
+        nd = int(out[i, 17])
+        if a_inf:
+            exp, want_nd = Bp, 0
+        elif Bp == As[i]:
+            exp, want_nd = As[i], 1                  # accumulator unchanged, caller doubles
+        else:
+            exp, want_nd = add(As[i], Bp), 0
+        assert nd == want_nd, i
+        if exp is None:
+            assert int(out[i, 16]) == 1, i
+        else:
+            assert int(out[i, 16]) == 0, i
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
+    assert sum(int(out[i, 17]) for i in range(n)) >= 30            # the a == b path ran
+    assert sum(int(out[i, 16]) for i in range(n) if i % 11 != 10) >= 30   # the a == -b path ran
+    out = gpu.selftest(OP_JAC26_DBL, a, b)
+    for i in range(n):
+        if i % 11 == 10:
+            assert int(out[i, 16]) == 1, i
+        else:
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == add(As[i], As[i]), i

@@ -43,6 +43,13 @@ enum SelftestOp : int {
     kOpFe26Sqr = 31,
     kOpFe26Norm = 32,
     kOpFe26FromTo = 33,
+    // Point formulas on 10 x 26-bit limbs.  Inputs: affine A (a = x || y) and
+    // B (b = x || y).  Item i uses A in Jacobian form with Z = i + 3, A at
+    // infinity when i % 11 == 10, and the addend flags i & 3 (kAddNeg,
+    // kAddBeta).  Outputs: affine x || y of A + B' (or 2A), [16] infinity,
+    // [17] the mixed addition's need-doubling flag.
+    kOpJac26Madd = 34,
+    kOpJac26Dbl = 35,
 };
 
 // Signature kinds of a verify batch.

@@ -407,6 +407,39 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         for (int l = 0; l < 8; ++l) o[l] = r.v[l];
         for (int l = 0; l < 10; ++l) o[8 + l] = x26.v[l];
     } break;
+    case kOpJac26Madd:
+    case kOpJac26Dbl: {
+        aff A, B;
+        for (int l = 0; l < 8; ++l) {
+            A.x.v[l] = a[16 * i + l];
+            A.y.v[l] = a[16 * i + 8 + l];
+            B.x.v[l] = b[16 * i + l];
+            B.y.v[l] = b[16 * i + 8 + l];
+        }
+        // A in Jacobian form with Z = i + 3: (x Z^2, y Z^3, Z)
+        fe z, z2, z3;
+        fe_set_u32(z, i + 3);
+        fe_sqr(z2, z);
+        fe_mul(z3, z2, z);
+        jac Aj;
+        fe_mul(Aj.x, A.x, z2);
+        fe_mul(Aj.y, A.y, z3);
+        Aj.z = z;
+        Aj.inf = 0;
+        jac26 a26, r26;
+        fe26_from_fe(a26.x, Aj.x);
+        fe26_from_fe(a26.y, Aj.y);
+        fe26_from_fe(a26.z, Aj.z);
+        a26.inf = 0;
+        if (i % 11 == 10) jac26_set_inf(a26);
+        bool nd = false;
+        if (op == kOpJac26Madd) jac26_madd(r26, a26, B, i & 3u, nd); else jac26_dbl(r26, a26);
+        jac R;
+        jac26_to_jac(R, r26);
+        o[16] = R.inf;
+        o[17] = nd;
+        if (!R.inf) { aff Ra; jac_to_aff(Ra, R); for (int l = 0; l < 8; ++l) { o[l] = Ra.x.v[l]; o[8 + l] = Ra.y.v[l]; } }
+    } break;
     default: break;
     }
 }

@@ -90,3 +90,27 @@ def test_msm_window_widths_at_2p18(gpu, c):
     rc, out = gpu.msm_rc(sc, pts, n, c)
     assert rc == UFSECP_OK
     assert out == compress(mul(e))
+
+
+@pytest.mark.parametrize("c", [8, 16])
+def test_msm_repeated_bases(gpu, c):
+    """Every term on the same base point: equal scalars put every term of a
+    window in one bucket, so the bucket sum adds a point to itself (the
+    mixed addition asks for a doubling), and scalars s and n - s cancel
+    (the sum passes through infinity and continues)."""
+    N = wl.N_ORDER
+    k = 0x1D3F5A7C9E0B2D4F6A8C0E1F3A5C7E9B1D3F5A7C9E0B2D4F6A8C0E1F3A5C7E9B % N
+    base = compress(mul(k))
+    s = 0x3C5E7091B3D5F7193B5D7F91B3D5F7193B5D7F91B3D5F7193B5D7F91B3D5F719 % N
+    cases = {
+        "equal": [s] * 1000,
+        "cancel": [s if i % 2 == 0 else N - s for i in range(1001)],
+        "mixed": [s, s, N - s, 2 * s % N, s, N - s, N - s] * 97,
+    }
+    for name, scal in cases.items():
+        sc = b"".join(v.to_bytes(32, "big") for v in scal)
+        pts = base * len(scal)
+        e = sum(scal) * k % N
+        rc, out = gpu.msm_rc(sc, pts, len(scal), c)
+        assert rc == UFSECP_OK, name
+        assert out == compress(mul(e)), name

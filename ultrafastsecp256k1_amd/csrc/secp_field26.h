@@ -15,9 +15,9 @@
 // l[i] <= m * (2^26 - 1) (i < 9) and l[9] <= m * (2^22 - 1).  Limbs are not
 // canonical; only fe26_to_fe and the zero tests normalise.
 //
-// Bounds (checked by exhaustive worst-case propagation over the doubling and
-// mixed-addition sequences below, which reach a fixed point of magnitudes
-// X <= 6.1, Y <= 10.1, Z <= 5.1):
+// Bounds (worst-case limb-bound propagation over the doubling and mixed
+// addition of secp_point26.h, which reach a fixed point of magnitudes
+// X <= 6.1, Y <= 10.1, Z <= 5.1; re-derived by tests/test_field26_bounds.py):
 //   fe26_mul / fe26_sqr : inputs of magnitude <= 20 (the 64-bit column sums
 //                         stay below 2^64); the loops use at most 16.  Output:
 //                         limbs <= 2^26 - 1 except l[2] <= 2^26 - 1 + 2^22,

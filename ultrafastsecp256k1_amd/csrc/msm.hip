@@ -52,44 +52,18 @@ UFS_DEV int msm_digit(const u32 k[8], int w, int c, int W, u32& carry) {
     return (int)raw;
 }
 
-// Counting sort of the bucket ids w * 2^(c-1) + |d| - 1 (counted in
-// k_msm_prepare, placed by k_msm_scatter).  Windows below the top one have
-// 2^(c-1) uniformly used buckets and count with plain global atomics.  The top
-// window's 2^tb buckets (up to 2^c, so it spans two groups of 2^(c-1)) are
-// few when tb is small and then receive a large share of all terms; their
-// counters share a few cache lines, so per-term global atomics on them
-// serialise (measured 2-6 ms at 2^20 terms).  When tb <= 8 the top window is
-// counted in LDS per block and each block reserves its range with one global
-// atomic per bucket.
-constexpr int kTermsPerThread = 2;
-constexpr int kTopLdsBits = 8;
-constexpr int kScatterGroup = 8;
-constexpr u32 kXcds = 8;   // XCDs of an MI355X; blocks are dealt to them round-robin
-
 // Scalars are split by GLV (k = k1 + lambda k2, |k1|, |k2| < 2^128): every
 // term becomes two of 128 bits, on P and phi(P) = (beta x, y) = lambda P.
 constexpr int kScalarBits = 128;
 
-UFS_DEV int msm_top_bits(int c, int W) { return kScalarBits - c * (W - 1); }
-
 // Parse + decompress every term (scalar < n, strict SEC1 point; the first
 // failing term decides the error, reported as 2i or 2i+1 through atomicMin),
-// split it into the two GLV terms 2i: (|k1|, +-P) and 2i+1: (|k2|, +-phi(P)),
-// and count their bucket ids.  The counting atomics overlap the square roots.
+// and split it into the two GLV terms 2i: (|k1|, +-P) and 2i+1: (|k2|, +-phi(P)).
 constexpr int kPrepTermsPerThread = 4;
 
 __global__ void __launch_bounds__(kMsmBlock)
 k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33, u32 n, aff* __restrict__ pts,
-              sc* __restrict__ scl, u32* __restrict__ counts, int c, int W, u32* __restrict__ err) {
-    __shared__ u32 tcnt[1 << kTopLdsBits];
-    const u32 B = 1u << (c - 1);
-    const int tb = msm_top_bits(c, W);
-    const bool lds = tb <= kTopLdsBits;
-    const u32 nkeys = 1u << (tb < kTopLdsBits ? tb : kTopLdsBits);
-    if (lds) {
-        for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) tcnt[t] = 0;
-        __syncthreads();
-    }
+              sc* __restrict__ scl, u32* __restrict__ err) {
     const size_t step = (size_t)gridDim.x * blockDim.x * kPrepTermsPerThread;
     for (size_t base = (size_t)blockIdx.x * blockDim.x * kPrepTermsPerThread; base < n; base += step) {
         for (int j = 0; j < kPrepTermsPerThread; ++j) {
@@ -143,120 +117,38 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
             pts[2 * i + 1] = Q;
             scl[2 * i] = k1;
             scl[2 * i + 1] = k2;
-            for (int h = 0; h < 2; ++h) {
-                const sc& kh = h ? k2 : k1;
-                u32 carry = 0;
-                for (int w = 0; w < W; ++w) {
-                    const int d = msm_digit(kh.v, w, c, W, carry);
-                    if (d == 0) continue;
-                    const u32 m = (u32)(d < 0 ? -d : d) - 1u;
-                    if (lds && w == W - 1) {
-                        atomicAdd(&tcnt[m], 1u);
-                    } else {
-                        atomicAdd(&counts[w * B + m], 1u);
-                    }
-                }
-            }
         }
-    }
-    if (lds) {
-        __syncthreads();
-        for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x)
-            if (tcnt[t]) atomicAdd(&counts[(W - 1) * B + t], tcnt[t]);
     }
 }
 
-// Sorted entry: bucket id in the high word, term index | sign bit in the low
-// word, written with one 8-byte store (one write transaction per scattered
-// entry instead of one per array).
+// Entry: bucket id in the high word, term index | sign bit in the low word,
+// one 8-byte store; sorting the entries by their high word groups them by
+// bucket.
 UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg) {
     return ((u64)bucket << 32) | (u64)(term | (neg ? 0x80000000u : 0u));
 }
 
-// Scatter of (bucket, term | sign) into bucket order; top-window positions
-// come from per-block LDS ranks plus one reserved range per bucket and block.
+// Unsorted entries in window-major order (one coalesced 8-byte store per
+// (half-term, window)), with bucket id `nbk` -- past every real bucket -- for a
+// zero digit; a radix sort on the bucket bits then groups them.  `total`
+// receives the number of non-zero digits (the sorted entries that count).
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_scatter(const sc* __restrict__ scl, u32 n, u32* __restrict__ cursor, u64* __restrict__ entries, int c,
-              int W) {
-    __shared__ u32 tcnt[1 << kTopLdsBits];
-    __shared__ u32 tbase[1 << kTopLdsBits];
+k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W, u32 nbk,
+              u32* __restrict__ total) {
     const u32 B = 1u << (c - 1);
-    const int tb = msm_top_bits(c, W);
-    const bool lds = tb <= kTopLdsBits;
-    const u32 nkeys = 1u << (tb < kTopLdsBits ? tb : kTopLdsBits);
-    // Blocks are dealt to the XCDs round-robin, so block b runs on XCD b % 8:
-    // it handles the windows w = b % 8 (mod 8) for its share of the terms, and
-    // every cursor of a window is then updated from one XCD only.
-    const int wfirst = (int)(blockIdx.x % kXcds);
-    const u32 slice = blockIdx.x / kXcds, nslices = gridDim.x / kXcds;
-    const size_t step = (size_t)nslices * blockDim.x * kTermsPerThread;
-    for (size_t base = (size_t)slice * blockDim.x * kTermsPerThread; base < n; base += step) {
-        if (lds) {
-            for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) tcnt[t] = 0;
-            __syncthreads();
-        }
-        u32 rank[kTermsPerThread];
-        int dtop[kTermsPerThread];
-#pragma unroll
-        for (int j = 0; j < kTermsPerThread; ++j) {
-            dtop[j] = 0;
-            rank[j] = 0;
-            const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
-            if (i >= n) continue;
-            const sc s = scl[i];
-            u32 carry = 0;
-            // groups of kScatterGroup windows: the digits first, then all the
-            // cursor reservations back to back (in flight together), then the
-            // stores; only this block's windows are placed
-            for (int w0 = 0; w0 < W; w0 += kScatterGroup) {
-                u32 bk[kScatterGroup], pos[kScatterGroup];
-                bool act[kScatterGroup], ng[kScatterGroup];
-#pragma unroll
-                for (int q = 0; q < kScatterGroup; ++q) {
-                    const int w = w0 + q;
-                    act[q] = false;
-                    ng[q] = false;
-                    bk[q] = 0;
-                    if (w < W) {
-                        const int d = msm_digit(s.v, w, c, W, carry);
-                        if (d != 0 && (w % kXcds) == wfirst) {
-                            bk[q] = w * B + (u32)(d < 0 ? -d : d) - 1u;
-                            ng[q] = d < 0;
-                            if (lds && w == W - 1) {
-                                dtop[j] = d;
-                                rank[j] = atomicAdd(&tcnt[bk[q] - w * B], 1u);
-                            } else {
-                                act[q] = true;
-                            }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kScatterGroup; ++q) pos[q] = act[q] ? atomicAdd(&cursor[bk[q]], 1u) : 0u;
-#pragma unroll
-                for (int q = 0; q < kScatterGroup; ++q)
-                    if (act[q]) entries[pos[q]] = msm_entry(bk[q], (u32)i, ng[q]);
-            }
-        }
-        if (lds) {
-            __syncthreads();
-            for (u32 t = threadIdx.x; t < nkeys; t += blockDim.x) {
-                const u32 cnt = tcnt[t];
-                tbase[t] = cnt ? atomicAdd(&cursor[(W - 1) * B + t], cnt) : 0u;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kTermsPerThread; ++j) {
-                const int d = dtop[j];
-                if (d == 0) continue;
-                const u32 m = (u32)(d < 0 ? -d : d) - 1u;
-                const u32 pos = tbase[m] + rank[j];
-                const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
-                entries[pos] = msm_entry((W - 1) * B + m, (u32)i, d < 0);
-            }
-            __syncthreads();
+    u32 cnt = 0;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
+        const sc s = scl[i];
+        u32 carry = 0;
+        for (int w = 0; w < W; ++w) {
+            const int d = msm_digit(s.v, w, c, W, carry);
+            const u32 key = d != 0 ? (u32)w * B + (u32)(d < 0 ? -d : d) - 1u : nbk;
+            entries[(size_t)w * n2 + i] = msm_entry(key, i, d < 0);
+            cnt += d != 0 ? 1u : 0u;
         }
     }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
+    if ((threadIdx.x & 63u) == 0 && cnt != 0) atomicAdd(total, cnt);
 }
 
 constexpr u32 kEmpty = 0xFFFFFFFFu;
@@ -560,32 +452,11 @@ hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
 }
 
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* counts, const MsmPlan& p, uint32_t* err, unsigned max_blocks,
-                              hipStream_t st) {
+                              uint32_t* err, unsigned max_blocks, hipStream_t st) {
     unsigned g = blocks_for(n, (unsigned)kMsmBlock * kPrepTermsPerThread);
     if (g > max_blocks) g = max_blocks;
     hipLaunchKernelGGL(k_msm_prepare, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points33, n, (aff*)pts, (sc*)scl,
-                       counts, p.c, p.windows, err);
-    return hipGetLastError();
-}
-
-// Grid of the counting-sort kernels: kTermsPerThread terms per lane per
-// pass, at most `max_blocks` blocks (fewer blocks = fewer reservations of
-// the top window's hot counters).
-static unsigned sort_grid(uint32_t n, unsigned max_blocks) {
-    const unsigned b = blocks_for(n, (unsigned)kMsmBlock * kTermsPerThread);
-    return b < max_blocks ? b : max_blocks;
-}
-
-// The scatter grid: kXcds blocks (one per window residue) per slice of terms.
-static unsigned scatter_grid(uint32_t n, unsigned max_blocks) {
-    return sort_grid(n, max_blocks / kXcds > 0 ? max_blocks / kXcds : 1) * kXcds;
-}
-
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint64_t* entries, const MsmPlan& p,
-                              unsigned max_blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_scatter, dim3(scatter_grid(n, max_blocks)), dim3(kMsmBlock), 0, st, (const sc*)scl, n,
-                       cursor, entries, p.c, p.windows);
+                       err);
     return hipGetLastError();
 }
 
@@ -640,10 +511,10 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
 }
 
 MsmEngine::~MsmEngine() {
-    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &counts_, &offsets_, &entries_,
+    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &entries_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
                            &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_,
-                           &rlc_in_, &rlc_terms_, &rlc_part_, &run_};
+                           &rlc_in_, &rlc_terms_, &rlc_part_, &run_, &sorted_, &sort_tmp_};
     for (DeviceBuffer* b : all) b->release();
 }
 
@@ -655,13 +526,14 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac);
     const size_t max_entries = 2 * n * (size_t)p.windows;   // two GLV terms per term
+    if (max_entries > 0x7FFFFFFFu)
+        return be_->fail(GpuError::BadInput, "MSM: too many (term, window) pairs for this window width");
     const uint32_t K = kSegChunk;
     const size_t chunks0 = (max_entries + K - 1) / K;
     const size_t slots0 = 2 * chunks0;
     hipError_t e;
     if ((e = pts_.ensure(2 * n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(2 * n * sizeof(sc))) != hipSuccess ||
-        (e = counts_.ensure((nbk + 1) * 4)) != hipSuccess || (e = offsets_.ensure((nbk + 1) * 4)) != hipSuccess ||
-        (e = entries_.ensure(max_entries * 8)) != hipSuccess ||
+        (e = entries_.ensure(max_entries * 8)) != hipSuccess || (e = sorted_.ensure(max_entries * 8)) != hipSuccess ||
         (e = bsum_.ensure(nbk * jsz)) != hipSuccess || (e = seg_.ensure(nseg * jsz)) != hipSuccess ||
         (e = red_a_.ensure(nseg * jsz)) != hipSuccess || (e = red_b_.ensure(nseg * jsz)) != hipSuccess ||
         (e = err_.ensure(4)) != hipSuccess || (e = slot_keys_[0].ensure(slots0 * 4)) != hipSuccess ||
@@ -670,10 +542,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         (e = sel_flags_.ensure(slots0)) != hipSuccess || (e = sel_count_.ensure(16)) != hipSuccess ||
         (e = run_.ensure(slots0 * 12)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
-    uint32_t* counts = (uint32_t*)counts_.ptr;
-    uint32_t* offsets = (uint32_t*)offsets_.ptr;
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
-    const unsigned sort_blocks = cus * 8, prep_blocks = cus * 4;
+    const unsigned prep_blocks = cus * 4;
     uint32_t* err = (uint32_t*)err_.ptr;
     uint32_t* sel_idx = (uint32_t*)sel_idx_.ptr;
     uint32_t* sel_count = (uint32_t*)sel_count_.ptr;
@@ -682,10 +552,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     uint32_t* pieces = run_start + slots0;
     uint32_t* npieces = sel_count + 1;
     size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0;
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, offsets, (int)(nbk + 1), st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
-                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
+    e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
+                                      (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
     if (e == hipSuccess)
         e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)slots0,
                                               st);
@@ -696,25 +564,37 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     if ((e = scan_tmp_.ensure(tmp_bytes + 16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
 
-    if ((e = launch_u32_fill(counts, 0u, nbk + 1, st)) != hipSuccess ||
-        (e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, counts, p, err,
-                                prep_blocks, st)) != hipSuccess)
+    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
+        (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks, st)) !=
+            hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
-    e = hipcub::DeviceScan::ExclusiveSum(scan_tmp_.ptr, tmp_bytes, counts, offsets, (int)(nbk + 1), st);
-    if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket scan", e);
-    e = hipMemcpyAsync(counts, offsets, (nbk + 1) * 4, hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return be_->fail(GpuError::Memory, "MSM cursor copy", e);
-    if ((e = launch_msm_scatter(scl_.ptr, (uint32_t)(2 * n), counts, (uint64_t*)entries_.ptr, p, sort_blocks, st)) !=
-        hipSuccess)
-        return be_->fail(GpuError::Launch, "MSM scatter", e);
+    // bucket order: unsorted window-major entries, then a radix sort on the
+    // bucket bits (zero digits carry bucket nbk and sort behind the rest)
+    uint32_t* total = sel_count + 2;
+    {
+        const uint32_t n2 = (uint32_t)(2 * n);
+        const int bits = 32 - __builtin_clz((unsigned)nbk);
+        size_t sort_bytes = 0;
+        e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)entries_.ptr,
+                                              (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM sort size query", e);
+        if ((e = sort_tmp_.ensure(sort_bytes + 16)) != hipSuccess)
+            return be_->fail(GpuError::Memory, "hipMalloc(MSM sort temp)", e);
+        if ((e = launch_u32_fill(total, 0u, 1, st)) != hipSuccess) return be_->fail(GpuError::Launch, "MSM total", e);
+        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl_.ptr,
+                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, (uint32_t)nbk, total);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
+        e = hipcub::DeviceRadixSort::SortKeys(sort_tmp_.ptr, sort_bytes, (const uint64_t*)entries_.ptr,
+                                              (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
+    }
     // bucket sums: empty buckets stay at infinity; level 0 over the sorted
-    // entries (total = offsets[nbk]), then chunked reductions of the compacted
+    // entries (the first `total` of them), then chunked reductions of the compacted
     // partials until a single chunk remains.
     hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, (jac*)bsum_.ptr,
                        (uint32_t)nbk);
     hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts_.ptr,
-                       (const uint64_t*)entries_.ptr, (const uint32_t*)(offsets + nbk), K, (uint32_t)chunks0,
+                       (const uint64_t*)sorted_.ptr, (const uint32_t*)total, K, (uint32_t)chunks0,
                        (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
     // run-aligned pieces: most buckets complete here; the pieces of long runs

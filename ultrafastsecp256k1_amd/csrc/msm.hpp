@@ -3,7 +3,7 @@
 // The reference's ufsecp_gpu_msm (src/gpu/src/gpu_backend_cuda.cu:1493-1572)
 // runs one full GLV scalar multiplication per term followed by a block tree
 // and a single-thread serial sum.  This engine is a bucket method with a
-// device-side counting sort instead:
+// device-side sort by bucket instead:
 //   1. parse + decompress: scalars < n and strict SEC1 points checked per term
 //      (first failing term decides the error, as in the reference CPU
 //      ufsecp_multi_scalar_mul, impl/ufsecp_taproot.cpp:957-991);
@@ -11,8 +11,9 @@
 //      into the terms (|k1|, +-P) and (|k2|, +-phi(P)), phi(P) = (beta x, y);
 //      c-bit digits of the 128-bit halves in W = ceil(128/c) windows: signed,
 //      |d| <= 2^(c-1), below the top window, unsigned (bits + carry, up to
-//      2^c) in the top one; histogram of (window, |d|) bucket ids, exclusive
-//      scan, scatter of (bucket, term index | sign) into bucket-sorted order;
+//      2^c) in the top one; one 8-byte entry (bucket id, term index | sign)
+//      per (half-term, window) written in window-major order, then a radix
+//      sort on the bucket bits (zero digits sort behind every bucket);
 //   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
 //      entries (mixed additions); the chunk-boundary partials of a bucket are
 //      summed by one lane per run of at most 16 (usually the last step), and
@@ -47,10 +48,7 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 
 // Device launchers (msm.hip)
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* counts, const MsmPlan& p, uint32_t* err, unsigned max_blocks,
-                              hipStream_t st);
-hipError_t launch_msm_scatter(const void* scl, uint32_t n, uint32_t* cursor, uint64_t* entries, const MsmPlan& p,
-                              unsigned max_blocks, hipStream_t st);
+                              uint32_t* err, unsigned max_blocks, hipStream_t st);
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
                              hipStream_t st);
@@ -78,9 +76,9 @@ private:
     GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                      int32_t* d_status, int c, hipStream_t st);
     RocmBackend* be_;
-    DeviceBuffer in_s_, in_p_, pts_, scl_, counts_, offsets_, entries_, bsum_, seg_, red_a_, red_b_, err_,
+    DeviceBuffer in_s_, in_p_, pts_, scl_, entries_, bsum_, seg_, red_a_, red_b_, err_,
         scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_flags_, sel_count_;
-    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_;
+    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_, sorted_, sort_tmp_;
 };
 
 }  // namespace ufs

@@ -130,13 +130,10 @@ UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg) {
 
 // Unsorted entries in window-major order (one coalesced 8-byte store per
 // (half-term, window)), with bucket id `nbk` -- past every real bucket -- for a
-// zero digit; a radix sort on the bucket bits then groups them.  `total`
-// receives the number of non-zero digits (the sorted entries that count).
+// zero digit; a radix sort on the bucket bits then groups them.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W, u32 nbk,
-              u32* __restrict__ total) {
+k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W, u32 nbk) {
     const u32 B = 1u << (c - 1);
-    u32 cnt = 0;
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
         const sc s = scl[i];
         u32 carry = 0;
@@ -144,11 +141,35 @@ k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int
             const int d = msm_digit(s.v, w, c, W, carry);
             const u32 key = d != 0 ? (u32)w * B + (u32)(d < 0 ? -d : d) - 1u : nbk;
             entries[(size_t)w * n2 + i] = msm_entry(key, i, d < 0);
-            cnt += d != 0 ? 1u : 0u;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
-    if ((threadIdx.x & 63u) == 0 && cnt != 0) atomicAdd(total, cnt);
+}
+
+// Number of real entries in the sorted array = index of the first entry
+// with bucket id >= nbk.  One wave, 64-ary search: each round probes 64
+// evenly spaced positions of [lo, hi) and keeps the gap holding the
+// boundary (every index below lo is a real entry, every index at or above
+// hi is not), so m entries take about log64(m) rounds of one load each.
+__global__ void k_msm_total(const u64* __restrict__ sorted, u32 m, u32 nbk, u32* __restrict__ total) {
+    const u32 lane = threadIdx.x;
+    u32 lo = 0, hi = m;
+    while (lo < hi) {
+        const u32 step = (hi - lo + 63u) / 64u;
+        const u32 pj = lo + lane * step;
+        const bool valid = pj < hi;
+        const bool past = valid && (u32)(sorted[pj] >> 32) >= nbk;
+        const u64 fmask = __ballot(past);
+        const u64 vmask = __ballot(valid);
+        if (fmask == 0) {
+            lo += (u32)(63 - __clzll((long long)vmask)) * step + 1u;
+        } else {
+            const u32 J = (u32)(__ffsll((long long)fmask) - 1);
+            const u32 pJ = lo + J * step;
+            if (J > 0) lo += (J - 1u) * step + 1u;
+            hi = pJ;
+        }
+    }
+    if (lane == 0) *total = lo;
 }
 
 constexpr u32 kEmpty = 0xFFFFFFFFu;
@@ -580,13 +601,15 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM sort size query", e);
         if ((e = sort_tmp_.ensure(sort_bytes + 16)) != hipSuccess)
             return be_->fail(GpuError::Memory, "hipMalloc(MSM sort temp)", e);
-        if ((e = launch_u32_fill(total, 0u, 1, st)) != hipSuccess) return be_->fail(GpuError::Launch, "MSM total", e);
         hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl_.ptr,
-                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, (uint32_t)nbk, total);
+                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, (uint32_t)nbk);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
         e = hipcub::DeviceRadixSort::SortKeys(sort_tmp_.ptr, sort_bytes, (const uint64_t*)entries_.ptr,
                                               (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
+        hipLaunchKernelGGL(k_msm_total, dim3(1), dim3(64), 0, st, (const uint64_t*)sorted_.ptr,
+                           (uint32_t)max_entries, (uint32_t)nbk, total);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entry count", e);
     }
     // bucket sums: empty buckets stay at infinity; level 0 over the sorted
     // entries (the first `total` of them), then chunked reductions of the compacted

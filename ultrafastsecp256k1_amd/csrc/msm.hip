@@ -422,15 +422,20 @@ k_msm_reduce(const jac* __restrict__ in, jac* __restrict__ out, u32 len, u32 out
 
 __global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u32* __restrict__ err,
                             u8* __restrict__ out33, int32_t* __restrict__ status) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    // one wave: every lane holds the same accumulator; the doublings spread
+    // their products over three lanes (jac26_dbl_lanes), lane 0 writes
+    if (blockIdx.x != 0) return;
+    const bool writer = threadIdx.x == 0;
     const u32 e = *err;
     if (e != 0xFFFFFFFFu) {
-        for (int i = 0; i < 33; ++i) out33[i] = 0;
-        *status = (e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */;
+        if (writer) {
+            for (int i = 0; i < 33; ++i) out33[i] = 0;
+            *status = (e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */;
+        }
         return;
     }
-    // One lane, so the time is the dependent-instruction latency of the
-    // (W - 1) c doublings: they run on 10 x 26-bit limbs, whose additions and
+    // The time is the dependent-instruction latency of the (W - 1) c
+    // doublings: they run on 10 x 26-bit limbs, whose additions and
     // subtractions are independent per limb instead of carry chains.
     jac acc;
     jac_add(acc, wsum[W - 1], wsum[W]);   // the top window's two bucket groups
@@ -440,10 +445,11 @@ __global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u3
         fe26_from_fe(a26.y, acc.y);
         fe26_from_fe(a26.z, acc.z);
         a26.inf = acc.inf;
-        for (u32 d = 0; d < c; ++d) jac26_dbl(a26, a26);
+        for (u32 d = 0; d < c; ++d) jac26_dbl_lanes(a26);
         jac26_to_jac(acc, a26);
         jac_add(acc, acc, wsum[w]);
     }
+    if (!writer) return;
     if (acc.inf) {
         for (int i = 0; i < 33; ++i) out33[i] = 0;
         *status = 7;   // ERR_ARITH: result is the point at infinity

@@ -151,4 +151,57 @@ UFS_DEV void jac26_madd_mem(jac26& out, const jac26& a, const aff* bp, u32 flags
     jac26_madd(out, a, b, flags, need_dbl);
 }
 
+UFS_DEV void fe26_shfl(fe26& r, const fe26& a, int src) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) r.v[i] = (u32)__shfl((int)a.v[i], src);
+}
+
+UFS_DEV void fe26_select(fe26& r, bool c, const fe26& a, const fe26& b) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) r.v[i] = c ? a.v[i] : b.v[i];
+}
+
+// a = 2a with the products of the doubling spread over lanes 0..2 of the
+// wave, for a single accumulator whose time is the latency of its dependent
+// instructions (every lane of the wave calls it with the same a and gets the
+// same result).  The operations and values are those of jac26_dbl:
+//   step 1 (one multiply, per lane):  X^2 = A (lane 0), Y^2 = B (1), Y Z (2)
+//   step 2 (one square, per lane):    (3A)^2 = F (0), B^2 = C (1), (X+B)^2 (2)
+//   step 3 (one multiply):            E (D - X3) with D, X3 from steps 1-2
+// so three multiplication latencies instead of seven, plus lane shuffles.
+UFS_DEV void jac26_dbl_lanes(jac26& a) {
+    const int lane = (int)(threadIdx.x & 63u);
+    fe26 p, q, r1, r2, B, C, T, A3, D, E, F, X3, Y3, Z3, t;
+    fe26_select(p, lane == 0, a.x, a.y);
+    fe26_select(q, lane == 0, a.x, a.y);
+    fe26_select(q, lane == 2, a.z, q);
+    fe26_mul(r1, p, q);                 // A | B | YZ
+    fe26_shfl(B, r1, 1);
+    fe26_mul3(E, r1);                   // 3A on lane 0
+    fe26_add(t, a.x, B);
+    fe26_select(p, lane == 0, E, r1);
+    fe26_select(p, lane == 2, t, p);
+    fe26_sqr(r2, p);                    // F | C | (X+B)^2
+    fe26_shfl(C, r2, 1);
+    fe26_shfl(T, r2, 2);
+    fe26_shfl(A3, r1, 0);               // A
+    fe26_shfl(F, r2, 0);
+    fe26_sub<2>(t, T, A3);
+    fe26_sub<2>(t, t, C);
+    fe26_add(t, t, t);
+    fe26_norm(D, t);                    // D = 2((X+B)^2 - A - C)
+    fe26_mul3(E, A3);                   // E = 3A
+    fe26_add(t, D, D);
+    fe26_sub<3>(X3, F, t);              // X3 = F - 2D
+    fe26_sub<5>(t, D, X3);
+    fe26_mul(t, E, t);
+    fe26_shl<3>(C, C);
+    fe26_sub<9>(Y3, t, C);              // Y3 = E(D - X3) - 8C
+    fe26_shfl(Z3, r1, 2);
+    fe26_add(Z3, Z3, Z3);               // Z3 = 2YZ
+    a.x = X3;
+    a.y = Y3;
+    a.z = Z3;
+}
+
 }  // namespace ufs

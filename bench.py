@@ -333,7 +333,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the verify path shards by item range with no communication, see DESIGN.md", file=sys.stderr)
+              "the verify path would shard by item range, one process per GPU, with no communication "
+              "between GPUs", file=sys.stderr)
         return 2
 
     from ultrafastsecp256k1_amd import UfsecpGpu

@@ -383,53 +383,4 @@ UFS_DEV void fe26_to_fe(fe& r, const fe26& a) {
     }
 }
 
-// r = a^((p+1)/4) in 8 x 32 form; returns true iff r^2 == a (a is a
-// quadratic residue).  The exponent chain (253 squarings, 13
-// multiplications) runs on 10 x 26-bit limbs, all inline: the chain's values
-// are products, so their magnitude stays at the multiply's output bound.
-// (Out-of-line squarings would keep the live values across calls and spill.)
-UFS_DEV bool fe_sqrt(fe& r, const fe& a8) {
-    fe26 a, x2, x3, x11, x22, x44, x88, t;
-    fe26_from_fe(a, a8);
-    auto sqr_n = [](fe26& v, int n) {
-        for (int i = 0; i < n; ++i) fe26_sqr(v, v);
-    };
-    fe26_sqr(t, a);
-    fe26_mul(x2, t, a);
-    fe26_sqr(t, x2);
-    fe26_mul(x3, t, a);
-    t = x3;
-    sqr_n(t, 3);
-    fe26_mul(t, t, x3);            // x6
-    sqr_n(t, 3);
-    fe26_mul(t, t, x3);            // x9
-    sqr_n(t, 2);
-    fe26_mul(x11, t, x2);
-    t = x11;
-    sqr_n(t, 11);
-    fe26_mul(x22, t, x11);
-    t = x22;
-    sqr_n(t, 22);
-    fe26_mul(x44, t, x22);
-    t = x44;
-    sqr_n(t, 44);
-    fe26_mul(x88, t, x44);
-    t = x88;
-    sqr_n(t, 88);
-    fe26_mul(t, t, x88);           // x176
-    sqr_n(t, 44);
-    fe26_mul(t, t, x44);           // x220
-    sqr_n(t, 3);
-    fe26_mul(t, t, x3);            // x223
-    sqr_n(t, 23);
-    fe26_mul(t, t, x22);
-    sqr_n(t, 6);
-    fe26_mul(t, t, x2);
-    sqr_n(t, 2);
-    fe26_to_fe(r, t);
-    fe c;
-    fe_sqr(c, r);
-    return fe_equal(c, a8);
-}
-
 }  // namespace ufs

@@ -18,7 +18,7 @@
 // Reference counterparts: src/cuda/include/secp256k1.cuh:1551-1782 (dbl/madd),
 // :2410-2491 (jacobian_add).
 #pragma once
-#include "secp_field26.h"
+#include "secp_field29.h"
 #include "secp_modinv.h"
 
 namespace ufs {

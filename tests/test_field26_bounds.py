@@ -10,12 +10,13 @@ correctness therefore rests on limb bounds, which this test derives:
   sequence of fe26_columns with 64-bit and 32-bit overflow checks (every
   intermediate is monotone in the inputs, so the all-maximum input bounds
   every other input);
-* limb-bound propagation through the doubling and the mixed addition exactly
-  as secp_point26.h writes them, iterated from the loop's entry values to a
-  fixed point: every multiply input stays within the allowed magnitude, every
-  fe26_norm input below 2^31, every limb below 2^32, and each fe26_sub<K> has
+* limb-bound propagation through the doubling, the mixed addition and the
+  general addition exactly as secp_point26.h writes them, iterated from the
+  loop's entry values to a fixed point: every multiply input stays within the
+  allowed magnitude, every fe26_norm input below 2^31, every limb below 2^32, and each fe26_sub<K> has
   K * p limb-wise at least its subtrahend's bound.  The K constants written
-  in secp_point26.h are asserted to be sufficient.
+  in secp_point26.h are asserted to be sufficient.  The general addition takes
+  both operands from the same bound set (loop outputs or exact conversions).
 """
 P = 2**256 - 2**32 - 977
 M26, M22, M32, M64 = (1 << 26) - 1, (1 << 22) - 1, (1 << 32) - 1, (1 << 64) - 1
@@ -28,6 +29,7 @@ LOOP_MUL_MAG = 16         # largest magnitude the loops feed it
 # fe26_sub<K> / fe26_neg<K> constants as written in secp_point26.h
 DBL_K = {"-A": 2, "-C": 2, "X3": 3, "D-X3": 5, "Y3": 9}
 MADD_K = {"h": 7, "s2-y": 11, "-j": 2, "X3": 3, "v-X3": 7, "Y3": 3, "-z1z1": 2, "Z3": 2, "neg y": 2}
+ADD_K = {"h": 2, "s2-s1": 2, "-j": 2, "X3": 3, "v-X3": 7, "Y3": 3}
 
 
 def test_constants():
@@ -173,6 +175,30 @@ def madd(X, Y, Z, bx, by):
     return X3, Y3, Z3
 
 
+def gadd(X1, Y1, Z1, X2, Y2, Z2):
+    """jac26_add (add-2007-bl with Z3 = 2 Z1 Z2 H)."""
+    z1z1 = mul(Z1, Z1)
+    z2z2 = mul(Z2, Z2)
+    u1 = mul(X1, z2z2)
+    u2 = mul(X2, z1z1)
+    s1 = mul(Y1, mul(Z2, z2z2))
+    s2 = mul(Y2, mul(Z1, z1z1))
+    h = norm(sub(u2, u1, ADD_K["h"]))
+    t = sub(s2, s1, ADD_K["s2-s1"])
+    rr = norm(add(t, t))
+    i4 = shl(mul(h, h), 2)
+    j = mul(h, i4)
+    v = mul(u1, i4)
+    t = sub(mul(rr, rr), j, ADD_K["-j"])
+    X3 = sub(t, add(v, v), ADD_K["X3"])
+    t = mul(rr, sub(v, X3, ADD_K["v-X3"]))
+    j = mul(s1, j)
+    Y3 = sub(t, add(j, j), ADD_K["Y3"])
+    z = mul(mul(Z1, Z2), h)
+    Z3 = add(z, z)
+    return X3, Y3, Z3
+
+
 def _vmax(*vs):
     return [max(c) for c in zip(*vs)]
 
@@ -188,8 +214,11 @@ def test_formulas_reach_a_fixed_point():
     for _ in range(10):
         d = dbl(X, Y, Z)
         m = madd(X, Y, Z, bx, by)
+        g = gadd(X, Y, Z, X, Y, Z)      # both operands anywhere in the state
         # also the madd's own special outputs: b itself (a = inf) and a itself
-        Xn, Yn, Zn = _vmax(X, d[0], m[0], bx), _vmax(Y, d[1], m[1], by), _vmax(Z, d[2], m[2], OUT)
+        Xn = _vmax(X, d[0], m[0], g[0], bx)
+        Yn = _vmax(Y, d[1], m[1], g[1], by)
+        Zn = _vmax(Z, d[2], m[2], g[2], OUT)
         if (Xn, Yn, Zn) == (X, Y, Z):
             break
         X, Y, Z = Xn, Yn, Zn

@@ -345,8 +345,15 @@ k_piece_reduce(const u32* __restrict__ idx, const u32* __restrict__ count_p, con
     const u32 beg = pieces[p];
     const u32 end = p + 1 < P ? pieces[p + 1] : E;
     const u32 key = ikeys[idx[beg]];
-    jac acc = ipart[idx[beg]];
-    for (u32 j = beg + 1; j < end; ++j) jac_add(acc, acc, ipart[idx[j]]);
+    jac26 a26;
+    jac26_from_jac(a26, ipart[idx[beg]]);
+    for (u32 j = beg + 1; j < end; ++j) {
+        jac26 b26;
+        jac26_from_jac(b26, ipart[idx[j]]);
+        jac26_add(a26, a26, b26);
+    }
+    jac acc;
+    jac26_to_jac(acc, a26);
     const bool whole = rstart[beg] == beg && (end == E || ikeys[idx[end]] != key);
     if (whole) {
         bsum[key] = acc;
@@ -358,21 +365,23 @@ k_piece_reduce(const u32* __restrict__ idx, const u32* __restrict__ count_p, con
 }
 
 // r = m * a for a small non-negative integer m (double-and-add).
-UFS_DEV void jac_mul_small(jac& r, const jac& a, u32 m) {
-    jac acc;
-    jac_set_inf(acc);
+UFS_DEV void jac26_mul_small(jac26& r, const jac26& a, u32 m) {
+    jac26 acc;
+    jac26_set_inf(acc);
     if (m != 0) {
         const int top = 31 - __builtin_clz(m);
         acc = a;
         for (int b = top - 1; b >= 0; --b) {
-            jac_dbl(acc, acc);
-            if ((m >> b) & 1u) jac_add(acc, acc, a);
+            jac26_dbl(acc, acc);
+            if ((m >> b) & 1u) jac26_add(acc, acc, a);
         }
     }
     r = acc;
 }
 
-// Lane (w, s): P = sum_{j<L} (s*L + j + 1) * B[w][s*L + j].
+// Lane (w, s): P = sum_{j<L} (s*L + j + 1) * B[w][s*L + j].  The lanes are
+// few (under one wave per SIMD) and each runs a chain of dependent point
+// additions, so the time follows the instruction count: 10 x 26-bit limbs.
 __global__ void __launch_bounds__(kMsmBlock)
 k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 G, u32 B, u32 L) {
     const u32 T = B / L;
@@ -382,45 +391,54 @@ k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 G, u32 B
     // group G - 1 holds magnitudes B + 1 .. 2B of the top window
     const u32 off = s * L + (w == G - 1 ? B : 0u);
     const jac* bb = bsum + (size_t)w * B + (size_t)s * L;
-    jac R, S;
-    jac_set_inf(R);
-    jac_set_inf(S);
+    jac26 R, S;
+    jac26_set_inf(R);
+    jac26_set_inf(S);
     for (int j = (int)L - 1; j >= 0; --j) {
-        jac_add(R, R, bb[j]);
-        jac_add(S, S, R);
+        jac26 Bj;
+        jac26_from_jac(Bj, bb[j]);
+        jac26_add(R, R, Bj);
+        jac26_add(S, S, R);
     }
-    jac M;
-    jac_mul_small(M, R, off);
-    jac_add(S, S, M);
-    seg[id] = S;
+    jac26 M;
+    jac26_mul_small(M, R, off);
+    jac26_add(S, S, M);
+    jac out;
+    jac26_to_jac(out, S);
+    seg[id] = out;
 }
 
 // Block tree sum of `len` points per group into out[g * out_len + blockIdx.x].
 __global__ void __launch_bounds__(kReduceBlock)
 k_msm_reduce(const jac* __restrict__ in, jac* __restrict__ out, u32 len, u32 out_len) {
-    __shared__ jac sh[kReduceBlock];
+    __shared__ jac26 sh[kReduceBlock];
     const u32 g = blockIdx.y, q = blockIdx.x, t = threadIdx.x;
     const u32 idx = q * kReduceBlock + t;
-    jac v;
+    jac26 v;
     if (idx < len) {
-        v = in[(size_t)g * len + idx];
+        jac26_from_jac(v, in[(size_t)g * len + idx]);
     } else {
-        jac_set_inf(v);
+        jac26_set_inf(v);
     }
     sh[t] = v;
     __syncthreads();
     for (u32 stride = kReduceBlock / 2; stride > 0; stride >>= 1) {
         if (t < stride) {
-            jac a = sh[t];
-            jac_add(a, a, sh[t + stride]);
+            jac26 a = sh[t];
+            const jac26 b = sh[t + stride];
+            jac26_add(a, a, b);
             sh[t] = a;
         }
         __syncthreads();
     }
-    if (t == 0) out[(size_t)g * out_len + q] = sh[0];
+    if (t == 0) {
+        jac r;
+        jac26_to_jac(r, sh[0]);
+        out[(size_t)g * out_len + q] = r;
+    }
 }
 
-__global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u32* __restrict__ err,
+__global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u32* __restrict__ err,
                             u8* __restrict__ out33, int32_t* __restrict__ status) {
     // one wave: every lane holds the same accumulator; the doublings spread
     // their products over three lanes (jac26_dbl_lanes), lane 0 writes
@@ -437,18 +455,17 @@ __global__ void k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u3
     // The time is the dependent-instruction latency of the (W - 1) c
     // doublings: they run on 10 x 26-bit limbs, whose additions and
     // subtractions are independent per limb instead of carry chains.
-    jac acc;
-    jac_add(acc, wsum[W - 1], wsum[W]);   // the top window's two bucket groups
+    jac26 a26, b26;
+    jac26_from_jac(a26, wsum[W - 1]);
+    jac26_from_jac(b26, wsum[W]);
+    jac26_add(a26, a26, b26);             // the top window's two bucket groups
     for (int w = (int)W - 2; w >= 0; --w) {
-        jac26 a26;
-        fe26_from_fe(a26.x, acc.x);
-        fe26_from_fe(a26.y, acc.y);
-        fe26_from_fe(a26.z, acc.z);
-        a26.inf = acc.inf;
         for (u32 d = 0; d < c; ++d) jac26_dbl_lanes(a26);
-        jac26_to_jac(acc, a26);
-        jac_add(acc, acc, wsum[w]);
+        jac26_from_jac(b26, wsum[w]);
+        jac26_add(a26, a26, b26);
     }
+    jac acc;
+    jac26_to_jac(acc, a26);
     if (!writer) return;
     if (acc.inf) {
         for (int i = 0; i < 33; ++i) out33[i] = 0;

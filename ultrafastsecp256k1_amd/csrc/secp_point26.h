@@ -151,6 +151,68 @@ UFS_DEV void jac26_madd_mem(jac26& out, const jac26& a, const aff* bp, u32 flags
     jac26_madd(out, a, b, flags, need_dbl);
 }
 
+UFS_DEV void jac26_from_jac(jac26& r, const jac& a) {
+    fe26_from_fe(r.x, a.x);
+    fe26_from_fe(r.y, a.y);
+    fe26_from_fe(r.z, a.z);
+    r.inf = a.inf;
+}
+
+// out = a + b for two Jacobian points (add-2007-bl: 11M + 5S, with
+// Z3 = 2 Z1 Z2 H).  Every case is handled: a = inf gives b, b = inf gives a,
+// a == b doubles (jac26_dbl), a == -b gives infinity.  `out` may alias an
+// operand.  Sub constants from the same limb-bound fixed point as the
+// doubling and the mixed addition, with both operands anywhere in it.
+UFS_DEV void jac26_add(jac26& out, const jac26& a, const jac26& b) {
+    fe26 z1z1, z2z2, u1, u2, s1, s2, h, rr, i4, j, v, t, t2;
+    fe26_sqr(z1z1, a.z);
+    fe26_sqr(z2z2, b.z);
+    fe26_mul(u1, a.x, z2z2);
+    fe26_mul(u2, b.x, z1z1);
+    fe26_mul(t, b.z, z2z2);
+    fe26_mul(s1, a.y, t);
+    fe26_mul(t, a.z, z1z1);
+    fe26_mul(s2, b.y, t);
+    fe26_sub<2>(t, u2, u1);
+    fe26_norm(h, t);           // H = U2 - U1
+    fe26_sub<2>(t, s2, s1);
+    fe26_add(t, t, t);
+    fe26_norm(rr, t);          // r = 2(S2 - S1)
+    fe26_sqr(t, h);
+    fe26_shl<2>(i4, t);        // I = (2H)^2
+    fe26_mul(j, h, i4);        // J = H I
+    fe26_mul(v, u1, i4);       // V = U1 I
+    jac26 r;
+    fe26_sqr(t, rr);
+    fe26_sub<2>(t, t, j);
+    fe26_add(t2, v, v);
+    fe26_sub<3>(r.x, t, t2);   // X3 = r^2 - J - 2V
+    fe26_sub<7>(t, v, r.x);
+    fe26_mul(t, rr, t);
+    fe26_mul(j, s1, j);
+    fe26_add(j, j, j);
+    fe26_sub<3>(r.y, t, j);    // Y3 = r (V - X3) - 2 S1 J
+    fe26_mul(t, a.z, b.z);
+    fe26_mul(t, t, h);
+    fe26_add(r.z, t, t);       // Z3 = 2 Z1 Z2 H
+    r.inf = 0;
+    // Rare and special cases behind a wave-uniform test (H and r are
+    // normalised: a zero H has limb 2 equal to 0 or 2^26 - 1).
+    if (__any(a.inf | b.inf | fe26_normed_maybe_zero(h))) {
+        const bool degen = fe26_normed_is_zero(h) & !a.inf & !b.inf;
+        const bool same = fe26_normed_is_zero(rr);
+        if (__any(degen & same)) {
+            jac26 d;
+            jac26_dbl(d, a);
+            if (degen & same) r = d;
+        }
+        if (degen & !same) jac26_set_inf(r);
+        if (b.inf) r = a;
+        if (a.inf) r = b;
+    }
+    out = r;
+}
+
 UFS_DEV void fe26_shfl(fe26& r, const fe26& a, int src) {
 #pragma unroll
     for (int i = 0; i < 10; ++i) r.v[i] = (u32)__shfl((int)a.v[i], src);

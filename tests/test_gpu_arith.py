@@ -236,7 +236,7 @@ def test_fe26_norm_and_round_trip(gpu):
         assert from_limbs(out[i, :8]) % P == x % P and from_limbs(out[i, :8]) < 2**256, i
 
 
-OP_JAC26_MADD, OP_JAC26_DBL = 34, 35
+OP_JAC26_MADD, OP_JAC26_DBL, OP_JAC26_ADD = 34, 35, 36
 
 
 def _aff_pack(points):
@@ -310,3 +310,41 @@ def test_jac26_madd_and_dbl_exceptional_cases(gpu):
             assert int(out[i, 16]) == 1, i
         else:
             assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == add(As[i], As[i]), i
+
+
+@pytest.mark.gpu
+def test_jac26_add_all_cases(gpu):
+    """The 10 x 26 general Jacobian addition (MSM window sums, tree reduce,
+    final) against the affine group law: B = A (doubling), B = -A (infinity),
+    either operand at infinity, and generic sums.  Item i uses Z = i + 3 for A
+    and Z = i + 5 for B; A = infinity when i % 11 == 10, B = infinity when
+    i % 13 == 12."""
+    rnd = random.Random(36)
+    As, Bs = [], []
+    n = 260
+    for i in range(n):
+        A = mul(rnd.randrange(1, N))
+        kind = i % 5
+        if kind == 0:
+            B = A
+        elif kind == 1:
+            B = (A[0], (P - A[1]) % P)
+        else:
+            B = mul(rnd.randrange(1, N))
+        As.append(A)
+        Bs.append(B)
+    out = gpu.selftest(OP_JAC26_ADD, _aff_pack(As), _aff_pack(Bs))
+    doubled = at_inf = 0
+    for i in range(n):
+        a = None if i % 11 == 10 else As[i]
+        b = None if i % 13 == 12 else Bs[i]
+        exp = add(a, b)
+        if a is not None and a == b:
+            doubled += 1
+        if exp is None:
+            at_inf += 1
+            assert int(out[i, 16]) == 1, i
+        else:
+            assert int(out[i, 16]) == 0, i
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
+    assert doubled >= 40 and at_inf >= 40

@@ -50,6 +50,7 @@ enum SelftestOp : int {
     // [17] the mixed addition's need-doubling flag.
     kOpJac26Madd = 34,
     kOpJac26Dbl = 35,
+    kOpJac26Add = 36,
 };
 
 // Signature kinds of a verify batch.

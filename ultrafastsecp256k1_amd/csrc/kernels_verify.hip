@@ -408,7 +408,8 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         for (int l = 0; l < 10; ++l) o[8 + l] = x26.v[l];
     } break;
     case kOpJac26Madd:
-    case kOpJac26Dbl: {
+    case kOpJac26Dbl:
+    case kOpJac26Add: {
         aff A, B;
         for (int l = 0; l < 8; ++l) {
             A.x.v[l] = a[16 * i + l];
@@ -433,7 +434,26 @@ k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __
         a26.inf = 0;
         if (i % 11 == 10) jac26_set_inf(a26);
         bool nd = false;
-        if (op == kOpJac26Madd) jac26_madd(r26, a26, B, i & 3u, nd); else jac26_dbl(r26, a26);
+        if (op == kOpJac26Madd) {
+            jac26_madd(r26, a26, B, i & 3u, nd);
+        } else if (op == kOpJac26Add) {
+            // B in Jacobian form with Z = i + 5; B = infinity when i % 13 == 12
+            fe zb, zb2, zb3;
+            fe_set_u32(zb, i + 5);
+            fe_sqr(zb2, zb);
+            fe_mul(zb3, zb2, zb);
+            jac Bj;
+            fe_mul(Bj.x, B.x, zb2);
+            fe_mul(Bj.y, B.y, zb3);
+            Bj.z = zb;
+            Bj.inf = 0;
+            jac26 b26;
+            jac26_from_jac(b26, Bj);
+            if (i % 13 == 12) jac26_set_inf(b26);
+            jac26_add(r26, a26, b26);
+        } else {
+            jac26_dbl(r26, a26);
+        }
         jac R;
         jac26_to_jac(R, r26);
         o[16] = R.inf;

@@ -213,9 +213,11 @@ UFS_DEV void jac26_add(jac26& out, const jac26& a, const jac26& b) {
     out = r;
 }
 
+// r = lane `src`'s a on every lane (src wave-uniform: v_readlane_b32, no LDS
+// round trip).
 UFS_DEV void fe26_shfl(fe26& r, const fe26& a, int src) {
 #pragma unroll
-    for (int i = 0; i < 10; ++i) r.v[i] = (u32)__shfl((int)a.v[i], src);
+    for (int i = 0; i < 10; ++i) r.v[i] = (u32)__builtin_amdgcn_readlane((int)a.v[i], src);
 }
 
 UFS_DEV void fe26_select(fe26& r, bool c, const fe26& a, const fe26& b) {

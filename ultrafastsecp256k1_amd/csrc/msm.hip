@@ -121,55 +121,41 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
     }
 }
 
-// Entry: bucket id in the high word, term index | sign bit in the low word,
-// one 8-byte store; sorting the entries by their high word groups them by
-// bucket.
-UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg) {
-    return ((u64)bucket << 32) | (u64)(term | (neg ? 0x80000000u : 0u));
+// Entry: bucket id in the high word, term index | sign bit (31) | skip bit
+// (30) in the low word, one 8-byte store.  Term indices stay below 2^29 (the
+// enqueue bounds 2N * W), so the flag bits never meet them.
+constexpr u32 kEntrySign = 0x80000000u;
+constexpr u32 kEntrySkip = 0x40000000u;
+constexpr u32 kEntryTerm = 0x3FFFFFFFu;
+
+UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg, bool skip) {
+    return ((u64)bucket << 32) | (u64)(term | (neg ? kEntrySign : 0u) | (skip ? kEntrySkip : 0u));
 }
 
 // Unsorted entries in window-major order (one coalesced 8-byte store per
-// (half-term, window)), with bucket id `nbk` -- past every real bucket -- for a
-// zero digit; a radix sort on the bucket bits then groups them.
+// (half-term, window)).  A zero digit becomes a skip entry in one of its own
+// window's buckets (chosen by the term index, so skips spread evenly and never
+// form long runs), which the accumulation passes over.
+//
+// Order: the sort keys on the low c bits of the bucket id only, which are
+// distinct for distinct buckets of one window ((w & 1) * B + b below the top
+// window; the top window's 2B ids are consecutive).  The sort is stable and
+// the input is window-major, so every bucket's entries still end up
+// contiguous -- buckets of different windows interleave, which nothing
+// downstream depends on -- and c bits (two 8-bit passes at c = 16) replace the
+// log2 of the bucket count (three passes).
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W, u32 nbk) {
+k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W) {
     const u32 B = 1u << (c - 1);
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
         const sc s = scl[i];
         u32 carry = 0;
         for (int w = 0; w < W; ++w) {
             const int d = msm_digit(s.v, w, c, W, carry);
-            const u32 key = d != 0 ? (u32)w * B + (u32)(d < 0 ? -d : d) - 1u : nbk;
-            entries[(size_t)w * n2 + i] = msm_entry(key, i, d < 0);
+            const u32 b = d != 0 ? (u32)(d < 0 ? -d : d) - 1u : (i & (B - 1u));
+            entries[(size_t)w * n2 + i] = msm_entry((u32)w * B + b, i, d < 0, d == 0);
         }
     }
-}
-
-// Number of real entries in the sorted array = index of the first entry
-// with bucket id >= nbk.  One wave, 64-ary search: each round probes 64
-// evenly spaced positions of [lo, hi) and keeps the gap holding the
-// boundary (every index below lo is a real entry, every index at or above
-// hi is not), so m entries take about log64(m) rounds of one load each.
-__global__ void k_msm_total(const u64* __restrict__ sorted, u32 m, u32 nbk, u32* __restrict__ total) {
-    const u32 lane = threadIdx.x;
-    u32 lo = 0, hi = m;
-    while (lo < hi) {
-        const u32 step = (hi - lo + 63u) / 64u;
-        const u32 pj = lo + lane * step;
-        const bool valid = pj < hi;
-        const bool past = valid && (u32)(sorted[pj] >> 32) >= nbk;
-        const u64 fmask = __ballot(past);
-        const u64 vmask = __ballot(valid);
-        if (fmask == 0) {
-            lo += (u32)(63 - __clzll((long long)vmask)) * step + 1u;
-        } else {
-            const u32 J = (u32)(__ffsll((long long)fmask) - 1);
-            const u32 pJ = lo + J * step;
-            if (J > 0) lo += (J - 1u) * step + 1u;
-            hi = pJ;
-        }
-    }
-    if (lane == 0) *total = lo;
 }
 
 constexpr u32 kEmpty = 0xFFFFFFFFu;
@@ -190,11 +176,10 @@ __global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p,
 // by the higher levels, so every lane does at most K additions whatever the
 // bucket sizes (the top window's buckets can hold half of all terms).
 __global__ void __launch_bounds__(kMsmBlock)
-k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* __restrict__ total_p, u32 K,
-       u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
+k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 K, u32 max_chunks,
+       jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= max_chunks) return;
-    const u32 E = *total_p;
     const u32 beg = j * K;
     jac hv, tv;
     jac_set_inf(hv);
@@ -210,14 +195,15 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* 
         // the running sum on 10 x 26-bit limbs, converted when a bucket ends
         jac26 acc;
         jac26_set_inf(acc);
-        aff nxt = pts[(u32)ent & 0x7fffffffu];
+        aff nxt = pts[(u32)ent & kEntryTerm];
         for (u32 e = beg; e < end; ++e) {
             const u32 k = (u32)(ent >> 32);
             aff P = nxt;
-            const bool neg = ((u32)ent >> 31) != 0;
+            const bool neg = ((u32)ent & kEntrySign) != 0;
+            const bool skip = ((u32)ent & kEntrySkip) != 0;
             if (e + 1 < end) {
                 ent = entries[e + 1];
-                nxt = pts[(u32)ent & 0x7fffffffu];
+                nxt = pts[(u32)ent & kEntryTerm];
             }
             if (k != cur) {
                 jac a8;
@@ -228,16 +214,17 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, const u32* 
                 jac26_set_inf(acc);
             }
             // inline mixed addition; a == P (possible for repeated points)
-            // doubles instead
+            // doubles instead; a skip entry (zero digit) leaves acc as it is
             jac26 t;
             bool nd;
             jac26_madd(t, acc, P, neg ? kAddNeg : 0u, nd);
+            nd &= !skip;
             if (__any(nd)) {
                 jac26 dd;
                 jac26_dbl(dd, acc);
                 if (nd) t = dd;
             }
-            acc = t;
+            if (!skip) acc = t;
         }
         jac a8;
         jac26_to_jac(a8, acc);
@@ -612,12 +599,11 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks, st)) !=
             hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
-    // bucket order: unsorted window-major entries, then a radix sort on the
-    // bucket bits (zero digits carry bucket nbk and sort behind the rest)
-    uint32_t* total = sel_count + 2;
+    // bucket order: unsorted window-major entries, then a stable radix sort
+    // on the c low bits of the bucket id (zero digits are skip entries)
     {
         const uint32_t n2 = (uint32_t)(2 * n);
-        const int bits = 32 - __builtin_clz((unsigned)nbk);
+        const int bits = p.c;   // bucket bits that differ within one window
         size_t sort_bytes = 0;
         e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)entries_.ptr,
                                               (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
@@ -625,22 +611,19 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         if ((e = sort_tmp_.ensure(sort_bytes + 16)) != hipSuccess)
             return be_->fail(GpuError::Memory, "hipMalloc(MSM sort temp)", e);
         hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl_.ptr,
-                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, (uint32_t)nbk);
+                           n2, (uint64_t*)entries_.ptr, p.c, p.windows);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
         e = hipcub::DeviceRadixSort::SortKeys(sort_tmp_.ptr, sort_bytes, (const uint64_t*)entries_.ptr,
                                               (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
-        hipLaunchKernelGGL(k_msm_total, dim3(1), dim3(64), 0, st, (const uint64_t*)sorted_.ptr,
-                           (uint32_t)max_entries, (uint32_t)nbk, total);
-        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entry count", e);
     }
-    // bucket sums: empty buckets stay at infinity; level 0 over the sorted
-    // entries (the first `total` of them), then chunked reductions of the compacted
+    // bucket sums: empty buckets stay at infinity; level 0 over all sorted
+    // entries (skips pass through), then chunked reductions of the compacted
     // partials until a single chunk remains.
     hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, (jac*)bsum_.ptr,
                        (uint32_t)nbk);
     hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts_.ptr,
-                       (const uint64_t*)sorted_.ptr, (const uint32_t*)total, K, (uint32_t)chunks0,
+                       (const uint64_t*)sorted_.ptr, (uint32_t)max_entries, K, (uint32_t)chunks0,
                        (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
     // run-aligned pieces: most buckets complete here; the pieces of long runs

@@ -12,8 +12,10 @@
 //      c-bit digits of the 128-bit halves in W = ceil(128/c) windows: signed,
 //      |d| <= 2^(c-1), below the top window, unsigned (bits + carry, up to
 //      2^c) in the top one; one 8-byte entry (bucket id, term index | sign)
-//      per (half-term, window) written in window-major order, then a radix
-//      sort on the bucket bits (zero digits sort behind every bucket);
+//      per (half-term, window) written in window-major order (a zero digit
+//      is a skip entry), then a stable radix sort on the c low bucket-id
+//      bits, which differ between the buckets of one window, so every
+//      bucket's entries end up contiguous;
 //   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
 //      entries (mixed additions); the chunk-boundary partials of a bucket are
 //      summed by one lane per run of at most 16 (usually the last step), and

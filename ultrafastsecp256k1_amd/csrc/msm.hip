@@ -224,7 +224,10 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
                 jac26_dbl(dd, acc);
                 if (nd) t = dd;
             }
-            if (!skip) acc = t;
+            if (__any(skip)) {
+                if (skip) t = acc;   // rare: only zero digits
+            }
+            acc = t;
         }
         jac a8;
         jac26_to_jac(a8, acc);

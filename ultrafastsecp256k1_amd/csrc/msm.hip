@@ -241,12 +241,6 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
     opart[2 * j + 1] = tv;
 }
 
-__global__ void __launch_bounds__(kMsmBlock)
-k_seg_flags(const u32* __restrict__ keys, u32 n, u8* __restrict__ flags) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = keys[i] != kEmpty;
-}
-
 // Higher levels: the same chunked run reduction over the compacted, still
 // bucket-ordered partials of the previous level (Jacobian additions).
 __global__ void __launch_bounds__(kMsmBlock)
@@ -477,6 +471,12 @@ __global__ void k_u32_fill(u32* p, u32 v, size_t n) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// Select predicate: slot i holds a partial (its key is not kEmpty).
+struct SlotNotEmpty {
+    const uint32_t* keys;
+    __host__ __device__ bool operator()(const uint32_t& i) const { return keys[i] != kEmpty; }
+};
+
 static inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
@@ -585,16 +585,20 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     uint32_t* run_start = run_heads + slots0;
     uint32_t* pieces = run_start + slots0;
     uint32_t* npieces = sel_count + 1;
-    size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0;
+    size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0, if_bytes = 0;
     e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
                                       (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceSelect::If(nullptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
+                                     sel_count, (int)slots0, SlotNotEmpty{(const uint32_t*)slot_keys_[0].ptr}, st);
     if (e == hipSuccess)
         e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)slots0,
                                               st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "scan size query", e);
     if (sel_bytes > tmp_bytes) tmp_bytes = sel_bytes;
     if (max_bytes > tmp_bytes) tmp_bytes = max_bytes;
-    sel_bytes = max_bytes = tmp_bytes;
+    if (if_bytes > tmp_bytes) tmp_bytes = if_bytes;
+    sel_bytes = max_bytes = if_bytes = tmp_bytes;
     if ((e = scan_tmp_.ensure(tmp_bytes + 16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
 
@@ -633,10 +637,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     // (in slot set 1) continue through the chunked levels below
     {
         const size_t S = slots0;
-        hipLaunchKernelGGL(k_seg_flags, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)slot_keys_[0].ptr, (uint32_t)S, (uint8_t*)sel_flags_.ptr);
-        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
-                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)S, st);
+        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
+                                     sel_count, (int)S, SlotNotEmpty{(const uint32_t*)slot_keys_[0].ptr}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
         hipLaunchKernelGGL(k_run_heads, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
                            (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,
@@ -661,10 +663,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     int lv = 1;
     for (;;) {
         const size_t chunks = (slots + K - 1) / K;
-        hipLaunchKernelGGL(k_seg_flags, dim3(blocks_for(slots, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)slot_keys_[lv].ptr, (uint32_t)slots, (uint8_t*)sel_flags_.ptr);
-        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
-                                          (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots, st);
+        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
+                                     sel_count, (int)slots, SlotNotEmpty{(const uint32_t*)slot_keys_[lv].ptr}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
         hipLaunchKernelGGL(k_seg, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
                            (const uint32_t*)sel_idx, (const uint32_t*)sel_count,

@@ -307,13 +307,6 @@ k_run_heads(const u32* __restrict__ idx, const u32* __restrict__ count_p, const 
 }
 
 __global__ void __launch_bounds__(kMsmBlock)
-k_piece_flags(const u32* __restrict__ rstart, const u32* __restrict__ count_p, u32 S, u8* __restrict__ flags) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S) return;
-    flags[i] = (i < *count_p && (i - rstart[i]) % kPieceLen == 0) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kMsmBlock)
 k_piece_reduce(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys,
                const jac* __restrict__ ipart, const u32* __restrict__ pieces, const u32* __restrict__ npieces_p,
                const u32* __restrict__ rstart, u32 S, jac* __restrict__ bsum, u32* __restrict__ okeys,
@@ -477,6 +470,16 @@ struct SlotNotEmpty {
     __host__ __device__ bool operator()(const uint32_t& i) const { return keys[i] != kEmpty; }
 };
 
+// Select predicate: compacted partial i starts a piece (every kPieceLen-th
+// partial of its run, counted from the run start).
+struct PieceStart {
+    const uint32_t* rstart;
+    const uint32_t* count;
+    __host__ __device__ bool operator()(const uint32_t& i) const {
+        return i < *count && (i - rstart[i]) % kPieceLen == 0;
+    }
+};
+
 static inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
@@ -547,7 +550,7 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
 MsmEngine::~MsmEngine() {
     DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &entries_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
-                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_flags_, &sel_count_,
+                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_count_,
                            &rlc_in_, &rlc_terms_, &rlc_part_, &run_, &sorted_, &sort_tmp_};
     for (DeviceBuffer* b : all) b->release();
 }
@@ -573,7 +576,7 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         (e = err_.ensure(4)) != hipSuccess || (e = slot_keys_[0].ensure(slots0 * 4)) != hipSuccess ||
         (e = slot_keys_[1].ensure(slots0 * 4)) != hipSuccess || (e = slot_part_[0].ensure(slots0 * jsz)) != hipSuccess ||
         (e = slot_part_[1].ensure(slots0 * jsz)) != hipSuccess || (e = sel_idx_.ensure(slots0 * 4)) != hipSuccess ||
-        (e = sel_flags_.ensure(slots0)) != hipSuccess || (e = sel_count_.ensure(16)) != hipSuccess ||
+        (e = sel_count_.ensure(16)) != hipSuccess ||
         (e = run_.ensure(slots0 * 12)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
@@ -586,8 +589,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     uint32_t* pieces = run_start + slots0;
     uint32_t* npieces = sel_count + 1;
     size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0, if_bytes = 0;
-    e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
-                                      (const uint8_t*)sel_flags_.ptr, sel_idx, sel_count, (int)slots0, st);
+    e = hipcub::DeviceSelect::If(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces, npieces,
+                                 (int)slots0, PieceStart{run_start, sel_count}, st);
     if (e == hipSuccess)
         e = hipcub::DeviceSelect::If(nullptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
                                      sel_count, (int)slots0, SlotNotEmpty{(const uint32_t*)slot_keys_[0].ptr}, st);
@@ -646,11 +649,8 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         e = hipcub::DeviceScan::InclusiveScan(scan_tmp_.ptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)S,
                                               st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM run starts", e);
-        hipLaunchKernelGGL(k_piece_flags, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)run_start, (const uint32_t*)sel_count, (uint32_t)S,
-                           (uint8_t*)sel_flags_.ptr);
-        e = hipcub::DeviceSelect::Flagged(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0),
-                                          (const uint8_t*)sel_flags_.ptr, pieces, npieces, (int)S, st);
+        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces,
+                                     npieces, (int)S, PieceStart{run_start, sel_count}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM piece list", e);
         hipLaunchKernelGGL(k_piece_reduce, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
                            (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,

@@ -79,7 +79,7 @@ private:
                      int32_t* d_status, int c, hipStream_t st);
     RocmBackend* be_;
     DeviceBuffer in_s_, in_p_, pts_, scl_, entries_, bsum_, seg_, red_a_, red_b_, err_,
-        scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_flags_, sel_count_;
+        scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_count_;
     DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_, sorted_, sort_tmp_;
 };
 

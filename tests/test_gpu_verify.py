@@ -157,3 +157,25 @@ def test_pipelined_opaque_rows(gpu):
     flat = rows.reshape(-1)[:(n - 1) * stride + 129].copy()
     v = gpu.ecdsa_verify_opaque_rows(flat, stride, n)
     assert np.array_equal(v, 1 - b.corrupted)
+
+
+@pytest.mark.gpu
+def test_schnorr_long_batch_pairs_share_inversion(gpu):
+    """A single BIP-340 batch longer than one pass of the resident grid (768
+    blocks x 4 waves x 64 items on MI355X): each lane verifies its items of
+    tiles t and t + W together, sharing the final inversion.  The last tile is
+    partial, so in its pair some lanes have a partner and some do not.
+    Verdicts against the generator's corruption flags (1% corrupted,
+    including keys that fail lift_x), per item and through the collect
+    variant."""
+    from ultrafastsecp256k1_amd import _buf
+    n = (1 << 19) + 77
+    b = wl.generate("schnorr", n, first=3000, threads=16)
+    v = gpu.schnorr_verify_batch(b.msgs, b.pubkeys, b.sigs)
+    assert np.array_equal(v, 1 - b.corrupted)
+    keybuf = np.full(n, 0xA5, np.uint8)
+    rc = gpu.lib.ufsecp_gpu_schnorr_verify_collect(gpu.ctx, _buf(b.msgs), _buf(b.pubkeys), _buf(b.sigs), n,
+                                                  _buf(keybuf))
+    assert rc == UFSECP_OK
+    assert np.array_equal(keybuf == 0, v == 1)
+    assert np.all(keybuf[v == 0] == 0xA5)

@@ -88,7 +88,7 @@ constexpr uint32_t kTileItems = 64;
 
 // Bytes of lane-private table space the verify kernel needs per grid lane
 // (twelve 64-byte affine entries).
-constexpr size_t kSlotBytes = 768;
+constexpr size_t kSlotBytes = 832;
 
 // Arguments of the persistent verify kernel.  Single mode (tiles == nullptr):
 // the tiles of `single`, single_tiles of them.  List mode: tiles[0 .. *ntiles_dev)

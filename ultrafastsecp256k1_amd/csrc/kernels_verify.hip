@@ -146,9 +146,10 @@ UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* s
     return ok && v;
 }
 
-// One BIP-340 verification of item i of batch d.
-UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot,
-                                 const aff* pts) {
+// BIP-340 verification of item i of batch d up to the y-parity test: R =
+// s G - e P in Jacobian form and whether every check but y(R) even passed.
+UFS_DEV bool schnorr_verify_head(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot,
+                                 const aff* pts, jac& R) {
     u32 m_w[8], px_w[8], r_w[8], s_w[8];
     load_be_words(m_w, d.msgs + 32ull * i);
     load_be_words(px_w, d.pubs + 32ull * i);
@@ -182,7 +183,6 @@ UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], co
     bool n1, n2;
     glv_split(k1, k2, n1, n2, b);
     build_ptable(slot, P);
-    jac R;
     ecmult_dual(R, slot, k1, n1, k2, n2, G, s);
     if (R.inf) return false;
     // x(R) == r without inversion (r < p): r * Z^2 == X
@@ -191,16 +191,61 @@ UFS_DEV bool schnorr_verify_item(const BatchDesc& d, u32 i, const u32 mid[8], co
     for (int k = 0; k < 8; ++k) rf.v[k] = rr[k];
     fe_sqr(z2, R.z);
     fe_mul(t, rf, z2);
-    bool v = fe_equal(t, R.x);
-    // y(R) even: y = Y / Z^3
-    fe zi, zi3, y;
-    fe_inv(zi, R.z);
+    return ok && fe_equal(t, R.x);
+}
+
+// y(R) even, with zi = 1/Z: y = Y zi^3.
+UFS_DEV bool y_is_even(const fe& Y, const fe& zi) {
+    fe zi3, y;
     fe_sqr(zi3, zi);
     fe_mul(zi3, zi3, zi);
-    fe_mul(y, R.y, zi3);
+    fe_mul(y, Y, zi3);
     fe_normalize(y);
-    v &= (y.v[0] & 1u) == 0;
-    return ok && v;
+    return (y.v[0] & 1u) == 0;
+}
+
+// BIP-340 verification of item i1 and, when `two`, item i2 of batch d, the
+// two sharing the final inversion (Montgomery's trick: 1/Z1 = Z2 / (Z1 Z2),
+// 1/Z2 = Z1 / (Z1 Z2)); a point at infinity, or the missing second item,
+// takes Z = 1 in the product (a point at infinity's verdict is false
+// already).  The head runs in a rolled loop: one inline copy of the
+// scalar multiplication serves both items.
+UFS_DEV void schnorr_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, const u32 mid[8], const gcomb& G,
+                                  aff* slot, const aff* pts, bool& v1, bool& v2) {
+    // the first item's Y and Z wait in the slot's last entry while the
+    // second item runs (no long-lived registers across the multiplication)
+    fe* stash = &slot[kSlotEntries - 1].x;
+    fe y1, z1, y2, z2;
+    fe_set_u32(z2, 1);
+    v2 = false;
+    const int m = two ? 2 : 1;
+#pragma unroll 1
+    for (int k = 0; k < m; ++k) {
+        jac R;
+        const bool v = schnorr_verify_head(d, k ? i2 : i1, mid, G, slot, pts, R);
+        fe z = R.z;
+        if (R.inf) fe_set_u32(z, 1);
+        if (k == 0) {
+            v1 = v;
+            store_fe(stash, R.y);
+            store_fe(stash + 1, z);
+        } else {
+            v2 = v;
+            y2 = R.y;
+            z2 = z;
+        }
+    }
+    load_fe(y1, stash);
+    load_fe(z1, stash + 1);
+    fe zz, inv, zi;
+    fe_mul(zz, z1, z2);
+    fe_inv(inv, zz);
+    fe_mul(zi, inv, z2);
+    v1 = v1 && y_is_even(y1, zi);
+    if (two) {
+        fe_mul(zi, inv, z1);
+        v2 = v2 && y_is_even(y2, zi);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -245,16 +290,28 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
             first = t * kTileItems;
             pts = reinterpret_cast<const aff*>(a.single_pts);
         }
-        t += nwaves;
+        // single BIP-340 batch: the lane's item of tile t + W joins the one
+        // of tile t (shared inversion); the tile loop then steps by 2W
+        const bool pairing = KIND == 1 && !list;
+        const u32 t2 = t + nwaves;
+        t += pairing ? 2 * nwaves : nwaves;
         const u32 i = first + lane;
         if (i >= d.count) continue;
+        const u32 i2 = t2 * kTileItems + lane;
+        const bool two = pairing && t2 < ntiles && i2 < d.count;
         const u32 kind = KIND == 2 ? d.kind : (u32)KIND;
-        const bool v = kind == kKindSchnorr ? schnorr_verify_item(d, i, mid, G, slot, pts)
-                                            : ecdsa_verify_item(d, i, G, slot, pts, kind == kKindEcdsaOpaque);
+        bool v, v2 = false;
+        if (kind == kKindSchnorr) {
+            schnorr_verify_items(d, i, i2, two, mid, G, slot, pts, v, v2);
+        } else {
+            v = ecdsa_verify_item(d, i, G, slot, pts, kind == kKindEcdsaOpaque);
+        }
         if (a.collect) {
             if (v) d.out[i] = 0;
+            if (two && v2) d.out[i2] = 0;
         } else {
             d.out[i] = v ? 1 : 0;
+            if (two) d.out[i2] = v2 ? 1 : 0;
         }
     }
 }

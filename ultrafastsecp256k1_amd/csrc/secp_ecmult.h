@@ -122,7 +122,7 @@ UFS_DEV void odd_digit(const u32 k[8], int i, int w, u32& idx, bool& neg) {
 // The entries for a window are loaded before its doublings, so the load
 // latency hides behind four doublings.
 // ---------------------------------------------------------------------------
-constexpr int kSlotEntries = 12;   // 768 bytes per lane
+constexpr int kSlotEntries = 13;   // 832 bytes per lane (entry 12: BIP-340 pair stash)
 
 UFS_DEV void store_fe(fe* p, const fe& a) {
     uint4* q = reinterpret_cast<uint4*>(p);

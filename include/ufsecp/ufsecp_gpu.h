@@ -199,6 +199,114 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ecrecover_batch(ufsecp_gpu_ctx* ctx, const 
 UFSECP_API ufsecp_error_t ufsecp_gpu_msm(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32, const uint8_t* points33,
                                          size_t n, uint8_t* out_result33);
 
+/* ---- operations this backend does not implement ------------------------- *
+ *
+ * The rest of the reference header, off the batch-verify / MSM path: secret-
+ * bearing operations (ECDH, BIP-324 AEAD, BIP-352 scanning, generator
+ * multiplication, which the reference runs on its constant-time kernel),
+ * byte hashing and the ZK / FROST verifiers.  Each is exported with the
+ * reference's signature so that callers written against the reference link
+ * and bind unchanged.  Each runs the reference's argument checks in the
+ * reference's order (src/cpu/src/ufsecp_gpu_impl.cpp:299-313, 495-538,
+ * 600-1127), so count == 0 is still OK with nothing written and a NULL buffer
+ * still NULL_ARG; a call that passes them clears its outputs and returns
+ * UFSECP_ERR_GPU_UNSUPPORTED -- the reference's code for an operation a
+ * backend lacks (src/gpu/include/gpu_backend.hpp:139-231). */
+
+#define UFSECP_ECDSA_SNARK_WITNESS_BYTES 760
+#define UFSECP_SCHNORR_SNARK_WITNESS_BYTES 472
+#define UFSECP_BIP352_SCAN_PLAN_BYTES 264
+
+/* Replaces ufsecp_gpu_generator_mul_batch (ufsecp_gpu.h:182-186). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_generator_mul_batch(ufsecp_gpu_ctx* ctx, const uint8_t* scalars32,
+                                                         size_t count, uint8_t* out_pubkeys33);
+
+/* Replaces ufsecp_gpu_ecdh_batch (ufsecp_gpu.h:364-369). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ecdh_batch(ufsecp_gpu_ctx* ctx, const uint8_t* privkeys32,
+                                                const uint8_t* peer_pubkeys33, size_t count,
+                                                uint8_t* out_secrets32);
+
+/* Replaces ufsecp_gpu_hash160_pubkey_batch (ufsecp_gpu.h:380-384). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_hash160_pubkey_batch(ufsecp_gpu_ctx* ctx, const uint8_t* pubkeys33,
+                                                          size_t count, uint8_t* out_hash160);
+
+/* Replaces ufsecp_gpu_tagged_hash (ufsecp_gpu.h:460-466). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_tagged_hash(ufsecp_gpu_ctx* ctx, const uint8_t* tag_hash32,
+                                                 const uint8_t* msgs, size_t msg_len, size_t n, uint8_t* out32);
+
+/* Replaces ufsecp_gpu_tagged_hash_var (ufsecp_gpu.h:496-503). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_tagged_hash_var(ufsecp_gpu_ctx* ctx, const uint8_t* tag_hash32,
+                                                     const uint8_t* msgs, const uint32_t* msg_lens, size_t stride,
+                                                     size_t n, uint8_t* out32);
+
+/* Replaces ufsecp_gpu_hash256 (ufsecp_gpu.h:516-521). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_hash256(ufsecp_gpu_ctx* ctx, const uint8_t* inputs, size_t input_len,
+                                             size_t n, uint8_t* out32);
+
+/* Replaces ufsecp_gpu_hash256_var (ufsecp_gpu.h:538-544). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_hash256_var(ufsecp_gpu_ctx* ctx, const uint8_t* inputs,
+                                                 const uint32_t* input_lens, size_t stride, size_t n,
+                                                 uint8_t* out32);
+
+/* Replaces ufsecp_gpu_frost_verify_partial_batch (ufsecp_gpu.h:569-580). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_frost_verify_partial_batch(
+    ufsecp_gpu_ctx* ctx, const uint8_t* z_i32, const uint8_t* D_i33, const uint8_t* E_i33, const uint8_t* Y_i33,
+    const uint8_t* rho_i32, const uint8_t* lambda_ie32, const uint8_t* negate_R, const uint8_t* negate_key,
+    size_t count, uint8_t* out_results);
+
+/* Replaces ufsecp_gpu_zk_knowledge_verify_batch (ufsecp_gpu.h:629-635). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_zk_knowledge_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* proofs64,
+                                                               const uint8_t* pubkeys65, const uint8_t* messages32,
+                                                               size_t count, uint8_t* out_results);
+
+/* Replaces ufsecp_gpu_zk_dleq_verify_batch (ufsecp_gpu.h:655-663). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_zk_dleq_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* proofs64,
+                                                          const uint8_t* G_pts65, const uint8_t* H_pts65,
+                                                          const uint8_t* P_pts65, const uint8_t* Q_pts65,
+                                                          size_t count, uint8_t* out_results);
+
+/* Replaces ufsecp_gpu_bulletproof_verify_batch (ufsecp_gpu.h:681-687). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_bulletproof_verify_batch(ufsecp_gpu_ctx* ctx, const uint8_t* proofs324,
+                                                              const uint8_t* commitments65,
+                                                              const uint8_t* H_generator65, size_t count,
+                                                              uint8_t* out_results);
+
+/* Replaces ufsecp_gpu_bip324_aead_encrypt_batch (ufsecp_gpu.h:710-718). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_bip324_aead_encrypt_batch(ufsecp_gpu_ctx* ctx, const uint8_t* keys32,
+                                                               const uint8_t* nonces12, const uint8_t* plaintexts,
+                                                               const uint32_t* sizes, uint32_t max_payload,
+                                                               size_t count, uint8_t* wire_out);
+
+/* Replaces ufsecp_gpu_bip324_aead_decrypt_batch (ufsecp_gpu.h:736-745). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_bip324_aead_decrypt_batch(ufsecp_gpu_ctx* ctx, const uint8_t* keys32,
+                                                               const uint8_t* nonces12, const uint8_t* wire_in,
+                                                               const uint32_t* sizes, uint32_t max_payload,
+                                                               size_t count, uint8_t* plaintext_out,
+                                                               uint8_t* out_valid);
+
+/* Replaces ufsecp_gpu_zk_ecdsa_snark_witness_batch (ufsecp_gpu.h:767-773). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_zk_ecdsa_snark_witness_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msg_hashes32,
+                                                                  const uint8_t* pubkeys33, const uint8_t* sigs64,
+                                                                  size_t count, uint8_t* out_witnesses);
+
+/* Replaces ufsecp_gpu_zk_schnorr_snark_witness_batch (ufsecp_gpu.h:794-800). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_zk_schnorr_snark_witness_batch(ufsecp_gpu_ctx* ctx, const uint8_t* msgs32,
+                                                                    const uint8_t* pubkeys_x32, const uint8_t* sigs64,
+                                                                    size_t count, uint8_t* out_witnesses);
+
+/* CPU-side plan for ufsecp_gpu_bip352_scan_batch: NULL_ARG on NULL buffers,
+ * otherwise the plan is zeroed and UFSECP_ERR_GPU_UNSUPPORTED returned (no
+ * secret key is read).  Replaces ufsecp_bip352_prepare_scan_plan
+ * (ufsecp_gpu.h:825-827). */
+UFSECP_API ufsecp_error_t ufsecp_bip352_prepare_scan_plan(const uint8_t scan_privkey32[32],
+                                                          uint8_t plan264_out[264]);
+
+/* Replaces ufsecp_gpu_bip352_scan_batch (ufsecp_gpu.h:861-867). */
+UFSECP_API ufsecp_error_t ufsecp_gpu_bip352_scan_batch(ufsecp_gpu_ctx* ctx, const uint8_t scan_privkey32[32],
+                                                       const uint8_t spend_pubkey33[33],
+                                                       const uint8_t* tweak_pubkeys33, size_t n_tweaks,
+                                                       uint64_t* prefix64_out);
+
 #ifdef __cplusplus
 }
 #endif

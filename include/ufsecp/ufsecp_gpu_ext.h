@@ -76,8 +76,10 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm(ufsecp_gpu_ctx* ctx, const uint8_t*
                                              size_t n, uint8_t* out_result33, int window_bits);
 
 /* Device-pointer MSM, asynchronous on `stream`: writes the 33-byte result to
- * d_out33 and the ABI status code (0, BAD_INPUT, BAD_PUBKEY or ERR_ARITH) to
- * *d_status, both device addresses. */
+ * d_out33 and the ABI status code (0, BAD_INPUT, BAD_PUBKEY or ERR_ARITH; or
+ * INTERNAL if a device-side bucket bound check ever fired) to *d_status, both
+ * device addresses.  Calls on different streams are ordered one after
+ * another (they share the context's MSM scratch). */
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
                                                     const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                                                     int32_t* d_status, int window_bits, void* stream);
@@ -103,6 +105,18 @@ UFSECP_API void* ufsecp_gpu_ext_stream(ufsecp_gpu_ctx* ctx);
 
 /* Synchronise the context's stream. */
 UFSECP_API ufsecp_error_t ufsecp_gpu_ext_synchronize(ufsecp_gpu_ctx* ctx);
+
+/* Lanes of the resident verification grid for a single batch of `kind` (0
+ * ECDSA, 1 BIP-340, 2 opaque ECDSA): lane l of the grid verifies items l,
+ * l + lanes, l + 2 lanes, ...; 0 for a NULL ctx or an unknown kind. */
+UFSECP_API uint32_t ufsecp_gpu_ext_verify_lanes(ufsecp_gpu_ctx* ctx, uint32_t kind);
+
+/* A new non-blocking HIP stream at the default priority on ctx's device, for
+ * callers that issue device-pointer calls on streams of their own;
+ * stream_synchronize waits for it, stream_destroy waits and releases it. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_stream_create(ufsecp_gpu_ctx* ctx, void** stream_out);
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_stream_synchronize(void* stream);
+UFSECP_API void ufsecp_gpu_ext_stream_destroy(void* stream);
 
 /* Arithmetic self-test: runs device operation `op` (field/scalar/point
  * primitives, codes in the library's test suite) on `count` records of 16

@@ -43,6 +43,7 @@ import ctypes
 import hashlib
 import json
 import os
+import random
 import re
 import sys
 import threading
@@ -54,7 +55,9 @@ ROOT = HERE.parent.parent
 REF_LIB = ROOT / "oracle" / "_ref" / "libufsecp_ref.so"
 REF_SRC = Path("/root/reference")
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(HERE.parent))
 
+import ec_math as ec  # noqa: E402  (textbook group law, for crafted items)
 from ultrafastsecp256k1_amd import workload as wl  # noqa: E402  (pure-Python spec helpers)
 
 N_ORDER = wl.N_ORDER
@@ -275,8 +278,13 @@ def edge_cases(ref: Ref) -> dict:
     ecdsa = []
 
     def add_e(name, m, pk, sg):
+        # opaque form: r and s byte-reversed (little-endian), verdict of the
+        # reference's libbitcoin row ABI (high-S accepted, r, s strict)
+        op = sg[:32][::-1] + sg[32:][::-1]
+        rc, ov = ref.verify_opaque_rows(m + pk + op, 129, 1)
+        assert rc == 0
         ecdsa.append({"name": name, "msg": m.hex(), "pub": pk.hex(), "sig": sg.hex(),
-                      "verdict": ref.ecdsa_verify(m, sg, pk)})
+                      "verdict": ref.ecdsa_verify(m, sg, pk), "opaque_verdict": ov[0]})
 
     add_e("valid", msg, pub, sig)
     add_e("high_s", msg, pub, b32(r) + b32(n - s))
@@ -305,24 +313,68 @@ def edge_cases(ref: Ref) -> dict:
     for kd in (1, 2, n - 1, n - 2, (n - 1) // 2):
         dd = b32(kd)
         add_e(f"key_{kd % 1000}", msg, ref.pubkey33(dd), ref.ecdsa_sign(msg, dd))
-    # r with x(R) in [n, p): craft (r, s) so that r = x(R) - n is exercised by the
-    # x-coordinate check; use u1 = 0 (msg = 0) so R = u2 * Q, choose Q = G and
-    # pick k with x(kG) >= n when it exists in a small search.
-    for k in range(1, 1 << 16):
-        pk_k = ref.pubkey33(b32(k))
-        X = int.from_bytes(pk_k[1:], "big")
-        if X >= n:
-            break
-    else:
-        k = None
-    if k is not None:
-        # sig with r = X - n, s chosen so that R' = (r/s)*G + (z/s)*G == kG with z = 0:
-        # u2 = r/s, u2*G = kG -> s = r/k mod n
-        rr = X - n
-        ss = rr * pow(k, -1, n) % n
-        if ss > n // 2:
-            ss = n - ss
-        add_e("x_ge_n_case", b32(0), ref.pubkey33(b32(1)), b32(rr) + b32(ss))
+    # Google Wycheproof ecdsa_secp256k1_sha256_bitcoin cases as transcribed in
+    # the reference's audit/test_wycheproof_ecdsa_bitcoin.cpp (message "123400",
+    # keys given as affine (wx, wy), DER signatures decoded to r, s):
+    #   tcId 346 (:181-211)  x(kG) >= n, r = x - n, s = 3        -> valid
+    #   tcId 347 (:215-248)  r = p - 3 (>= n), s = 3, same key    -> invalid
+    #   tcId 348 (:250-289)  r = n - 2, s = 3                     -> valid
+    #   tcId 351 (:291-323)  r = 1, s = 1                         -> valid
+    #   tcId 386 (:506-540)  u1 G + u2 Q hits infinity            -> invalid
+    wmsg = hashlib.sha256(b"123400").digest()
+
+    def wkey(wx, wy):
+        return bytes([2 | (int(wy, 16) & 1)]) + bytes.fromhex(wx)
+
+    k346 = wkey("07310f90a9eae149a08402f54194a0f7b4ac427bf8d9bd6c7681071dc47dc362",
+                "26a6d37ac46d61fd600c0bf1bff87689ed117dda6b0e59318ae010a197a26ca0")
+    add_e("wycheproof_346_x_ge_n", wmsg, k346, b32(0x014551231950B75FC4402DA1722FC9BAEB) + b32(3))
+    add_e("wycheproof_347_r_p_minus_3", wmsg, k346, b32(p - 3) + b32(3))
+    add_e("wycheproof_348_r_n_minus_2", wmsg,
+          wkey("bc97e7585eecad48e16683bc4091708e1a930c683fc47001d4b383594f2c4e22",
+               "705989cf69daeadd4e4e4b8151ed888dfec20fb01728d89d56b3f38f2ae9c8c5"), b32(n - 2) + b32(3))
+    add_e("wycheproof_351_r1_s1", wmsg,
+          wkey("1877045be25d34a1d0600f9d5c00d0645a2a54379b6ceefad2e6bf5c2a3352ce",
+               "821a532cc1751ee1d36d41c3d6ab4e9b143e44ec46d73478ea6a79a5c0e54159"), b32(1) + b32(1))
+    add_e("wycheproof_386_infinity", wmsg,
+          wkey("d533b789a4af890fa7a82a1fae58c404f9a62a50b49adafab349c513b4150874",
+               "01b4171b803e76b34a9861e10f7bc289a066fd01bd29f84c987a10a5fb18c2d4"),
+          b32(0x7FFFFFFFFFFFFFFFFFFFFFFFFFFFFFFF5D576E7357A4501DDFE92F46681B20A0)
+          + b32(0x55555555555555555555555555555554E8E4F44CE51835693FF0CA2EF01215C0))
+
+    # Crafted x(R) in [n, p) items.  For any point R and any (z, s), the key
+    # Q = r^-1 (s R - z G) gives u1 G + u2 Q = (z/s) G + (r/s) Q = R, so
+    # (r, s) verifies on z exactly when x(R) == r (mod n) by the reference's
+    # rule (ecdsa.cpp:764-811): r = x(R) - n takes its r + n branch.  No
+    # discrete logarithm of R is needed.
+    rng = random.Random(0xEC0A)
+
+    def craft(xr, odd, r, name, z=None, s=None, high_s=False):
+        R = ec.lift_x(xr, odd)
+        assert R is not None
+        z = rng.getrandbits(256) % n if z is None else z
+        s = (rng.getrandbits(255) % (n // 2)) or 1 if s is None else s
+        Q = ec.mul(pow(r, -1, n), ec.add(ec.mul(s, R), ec.mul(n - z % n)))
+        add_e(name, b32(z), ec.compress(Q), b32(r) + b32(n - s if high_s else s))
+
+    lifts = [x for x in range(n + 1, n + 200) if ec.lift_x(x) is not None]
+    for j, xr in enumerate(lifts[:3]):
+        craft(xr, j & 1, xr - n, f"xr_ge_n_valid_{j}")
+    craft(lifts[3], 0, lifts[3] - n, "xr_ge_n_msg_zero", z=0)
+    craft(lifts[4], 1, lifts[4] - n, "xr_ge_n_high_s", high_s=True)
+    xtop = next(x for x in range(p - 1, p - 1000, -1) if ec.lift_x(x) is not None)
+    craft(xtop, 1, xtop - n, "xr_ge_n_top")          # the largest r that takes the r + n branch
+    craft(lifts[5], 0, lifts[5] - n + 1, "xr_ge_n_r_off_by_one")
+    # guard of the r + n branch: r + n >= p must not be compared.  R has a small
+    # x = X, r = X + p - n < n, so (r + n) mod p == X but x(R) mod n != r.
+    xsmall = next(x for x in range(2, 200) if ec.lift_x(x) is not None)
+    craft(xsmall, 0, xsmall + p - n, "r_plus_n_ge_p_guard")
+    craft(xsmall, 1, xsmall, "xr_small_valid")
+    # R' = infinity: key d G and z = -r d (mod n) give u1 G + u2 Q = ((z + r d)/s) G = O
+    for j, kd in enumerate((1, 0x5EED5EED)):
+        sg = ref.ecdsa_sign(msg, b32(kd))
+        rr = int.from_bytes(sg[:32], "big")
+        add_e(f"r_prime_infinity_{j}", b32((-rr * kd) % n), ref.pubkey33(b32(kd)), sg)
 
     schnorr = []
     aux = b"\x11" * 32
@@ -349,6 +401,19 @@ def edge_cases(ref: Ref) -> dict:
     for kd in (1, 2, n - 1, (n - 1) // 2):
         dd = b32(kd)
         add_s(f"key_{kd % 1000}", msg, ref.xonly(dd), ref.schnorr_sign(msg, dd, aux))
+    # R = s G - e P = infinity: P = d' G with even y (x-only key) and s = e d'
+    # (mod n), e = tagged_hash("BIP0340/challenge", r || P.x || m) mod n.  d' = 1
+    # (P = G) and a random key; r is the x of a valid point in one case and of
+    # G in the other.
+    tag = hashlib.sha256(b"BIP0340/challenge").digest()
+    for j, kd in enumerate((1, 0xB1B340)):
+        pt = ec.mul(kd)
+        dd = kd if pt[1] % 2 == 0 else n - kd
+        pxk = pt[0].to_bytes(32, "big")
+        rb = ssig[:32] if j else pxk
+        mm = hashlib.sha256(b"inf%d" % j).digest()
+        e = int.from_bytes(hashlib.sha256(tag + tag + rb + pxk + mm).digest(), "big") % n
+        add_s(f"r_infinity_{j}", mm, pxk, rb + b32(e * dd % n))
 
     # MSM cases (reference CPU ufsecp_multi_scalar_mul codes and outputs)
     msm = []

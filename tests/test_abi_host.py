@@ -115,3 +115,81 @@ def test_batch_check_null_context():
     ok = ctypes.c_int(5)
     buf = (ctypes.c_uint8 * 128)()
     assert lib.ufsecp_gpu_ext_schnorr_batch_check(None, buf, buf, buf, ctypes.c_size_t(1), ctypes.byref(ok)) == 1
+
+
+REF_GPU_HEADER = pathlib.Path("/root/reference/include/ufsecp/ufsecp_gpu.h")
+
+
+@pytest.mark.skipif(not REF_GPU_HEADER.exists(), reason="reference sources not mounted")
+def test_every_reference_declaration_is_declared_and_exported():
+    """Every UFSECP_API function of the reference's include/ufsecp/ufsecp_gpu.h
+    is declared in include/ufsecp/ufsecp_gpu.h and exported by the library, so
+    a caller compiled or bound against the reference header links here."""
+    ref = set(re.findall(r"UFSECP_API\s+[^;(]*?\b(ufsecp_\w+)\s*\(", REF_GPU_HEADER.read_text(), re.S))
+    assert len(ref) >= 41
+    assert not ref - declared_symbols()
+    lib = ufs.load_library()
+    assert [n for n in sorted(ref) if not hasattr(lib, n)] == []
+
+
+def test_unsupported_ops_checks_without_context():
+    """Operations off the verify / MSM path keep the reference's argument
+    checks (src/cpu/src/ufsecp_gpu_impl.cpp:299-1127): NULL ctx -> NULL_ARG,
+    except the SNARK witness calls, which test count first (count 0 -> OK,
+    count > 2^26 -> BAD_INPUT, then NULL -> NULL_ARG)."""
+    lib = ufs.load_library()
+    b = (ctypes.c_uint8 * 1024)()
+    one, big = ctypes.c_size_t(1), ctypes.c_size_t((1 << 26) + 1)
+    assert lib.ufsecp_gpu_generator_mul_batch(None, b, one, b) == 1
+    assert lib.ufsecp_gpu_ecdh_batch(None, b, b, one, b) == 1
+    assert lib.ufsecp_gpu_hash160_pubkey_batch(None, b, one, b) == 1
+    assert lib.ufsecp_gpu_tagged_hash(None, b, b, ctypes.c_size_t(32), one, b) == 1
+    assert lib.ufsecp_gpu_tagged_hash_var(None, b, b, b, ctypes.c_size_t(32), one, b) == 1
+    assert lib.ufsecp_gpu_hash256(None, b, ctypes.c_size_t(64), one, b) == 1
+    assert lib.ufsecp_gpu_hash256_var(None, b, b, ctypes.c_size_t(64), one, b) == 1
+    assert lib.ufsecp_gpu_frost_verify_partial_batch(None, b, b, b, b, b, b, b, b, one, b) == 1
+    assert lib.ufsecp_gpu_zk_knowledge_verify_batch(None, b, b, b, one, b) == 1
+    assert lib.ufsecp_gpu_zk_dleq_verify_batch(None, b, b, b, b, b, one, b) == 1
+    assert lib.ufsecp_gpu_bulletproof_verify_batch(None, b, b, b, one, b) == 1
+    assert lib.ufsecp_gpu_bip324_aead_encrypt_batch(None, b, b, b, b, ctypes.c_uint32(8), one, b) == 1
+    assert lib.ufsecp_gpu_bip324_aead_decrypt_batch(None, b, b, b, b, ctypes.c_uint32(8), one, b, b) == 1
+    assert lib.ufsecp_gpu_bip352_scan_batch(None, b, b, b, one, b) == 1
+    for fn in (lib.ufsecp_gpu_zk_ecdsa_snark_witness_batch, lib.ufsecp_gpu_zk_schnorr_snark_witness_batch):
+        assert fn(None, b, b, b, ctypes.c_size_t(0), b) == 0
+        assert fn(None, b, b, b, big, None) == 5
+        assert fn(None, b, b, b, one, None) == 1
+    plan = (ctypes.c_uint8 * 264)(*([0xAB] * 264))
+    assert lib.ufsecp_bip352_prepare_scan_plan(None, plan) == 1
+    assert lib.ufsecp_bip352_prepare_scan_plan(b, plan) == ufs.UFSECP_ERR_GPU_UNSUPPORTED
+    assert bytes(plan) == bytes(264)
+
+
+AUDIT_BIN = ROOT / "oracle" / "_ref"
+
+
+def run_reference_audit(name):
+    """Runs one of the reference's GPU C-ABI audit programs (compiled from its
+    audit/ sources by oracle/ref.mk against this library); returns (passed,
+    failed, FAIL lines)."""
+    import os
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        env = dict(os.environ, SECP256K1_CACHE_DIR=tmp)
+        res = subprocess.run([str(AUDIT_BIN / name)], cwd=tmp, env=env, capture_output=True, text=True, timeout=300)
+    m = re.search(r"Results: (\d+) passed, (\d+) failed", res.stdout)
+    assert m, res.stdout[-2000:] + res.stderr[-2000:]
+    fails = [ln.strip() for ln in res.stdout.splitlines() if "FAIL" in ln]
+    return int(m.group(1)), int(m.group(2)), fails
+
+
+@pytest.mark.skipif(not (AUDIT_BIN / "test_gpu_abi_gate").exists(), reason="reference audit programs not built")
+def test_reference_abi_audit_programs_host():
+    """The reference's audit/test_gpu_host_api_negative.cpp passes in full
+    against this library; audit/test_gpu_abi_gate.cpp fails only its check
+    that every backend id is 1..3 (this library adds id 4, ROCm).  Without a
+    GPU both skip their context sections."""
+    p, f, _ = run_reference_audit("test_gpu_host_api_negative")
+    assert f == 0 and p >= 30
+    p, f, fails = run_reference_audit("test_gpu_abi_gate")
+    assert f == 1 and fails == ["FAIL: backend id in range [1,3]"]

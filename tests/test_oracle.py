@@ -44,6 +44,74 @@ def test_edge_cases(golden, kind):
     fn = orc.ecdsa_verify if kind == "ecdsa" else orc.schnorr_verify
     for c in golden["edge_cases"][kind]:
         assert fn(bytes.fromhex(c["msg"]), bytes.fromhex(c["sig"]), bytes.fromhex(c["pub"])) == c["verdict"], c["name"]
+        if kind == "ecdsa":   # the same item as a libbitcoin opaque signature (r, s little-endian)
+            sg = bytes.fromhex(c["sig"])
+            v = orc.ecdsa_verify_opaque(bytes.fromhex(c["msg"]), sg[:32][::-1] + sg[32:][::-1], bytes.fromhex(c["pub"]))
+            assert v == c["opaque_verdict"], c["name"]
+
+
+def _textbook_ecdsa(msg, sig, pub):
+    """(verdict, R') by the textbook group law with the reference CPU rules
+    (src/cpu/src/impl/ufsecp_ecdsa.cpp:244-275: r, s in [1, n), s <= n/2;
+    ecdsa.cpp:891-927: prefix 02/03, x < p, on the curve; ecdsa.cpp:764-811:
+    R' finite and x(R') == r mod n)."""
+    r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+    Q = lift_x(int.from_bytes(pub[1:], "big"), pub[0] == 3) if pub[0] in (2, 3) else None
+    if not (0 < r < N and 0 < s <= N // 2) or Q is None:
+        return 0, None
+    w = pow(s, -1, N)
+    R = add_pts(mul(int.from_bytes(msg, "big") * w % N), mul(r * w % N, Q))
+    return int(R is not None and R[0] % N == r), R
+
+
+def _textbook_schnorr(msg, sig, px):
+    """(verdict, R) of BIP-340 by the textbook group law (schnorr.cpp:355-371,
+    745-806: r < p, 0 < s < n, x-only key lifts with even y, R = sG - eP
+    finite with even y and x(R) == r)."""
+    r, s = int.from_bytes(sig[:32], "big"), int.from_bytes(sig[32:], "big")
+    Pt = lift_x(int.from_bytes(px, "big"))
+    if r >= P or not 0 < s < N or Pt is None:
+        return 0, None
+    tag = hashlib.sha256(b"BIP0340/challenge").digest()
+    e = int.from_bytes(hashlib.sha256(tag + tag + sig[:32] + px + msg).digest(), "big") % N
+    R = add_pts(mul(s), mul(N - e, Pt))
+    return int(R is not None and R[1] % 2 == 0 and R[0] == r), R
+
+
+def test_edge_cases_textbook_and_branches(golden):
+    """Every ECDSA / BIP-340 edge case has the verdict the textbook group law
+    gives, and each crafted item really takes the branch it is named for:
+    x(R') in [n, p) with r = x(R') - n, the r + n >= p guard, R' = infinity,
+    BIP-340 R = infinity."""
+    by = {}
+    for c in golden["edge_cases"]["ecdsa"]:
+        m, sg, pk = (bytes.fromhex(c[k]) for k in ("msg", "sig", "pub"))
+        v, R = _textbook_ecdsa(m, sg, pk)
+        assert v == c["verdict"], c["name"]
+        by[c["name"]] = (int.from_bytes(sg[:32], "big"), R)
+    for name in ("wycheproof_346_x_ge_n", "xr_ge_n_valid_0", "xr_ge_n_valid_1", "xr_ge_n_valid_2",
+                 "xr_ge_n_msg_zero", "xr_ge_n_top"):
+        r, R = by[name]
+        assert R[0] >= N and R[0] - N == r, name
+    r, R = by["xr_ge_n_top"]
+    assert r + N < P and P - (r + N) < 1000          # the largest x < p that lifts
+    r, R = by["xr_ge_n_r_off_by_one"]
+    assert R[0] >= N and R[0] - N == r - 1
+    r, R = by["r_plus_n_ge_p_guard"]
+    assert r < N and r + N >= P and (r + N) % P == R[0]   # would match if r + n were compared mod p
+    for name in ("wycheproof_386_infinity", "r_prime_infinity_0", "r_prime_infinity_1"):
+        assert by[name][1] is None, name
+    assert by["wycheproof_347_r_p_minus_3"][0] >= N
+    hs = next(c for c in golden["edge_cases"]["ecdsa"] if c["name"] == "xr_ge_n_high_s")
+    m, sg, pk = (bytes.fromhex(hs[k]) for k in ("msg", "sig", "pub"))
+    s_low = N - int.from_bytes(sg[32:], "big")
+    assert _textbook_ecdsa(m, sg[:32] + s_low.to_bytes(32, "big"), pk)[0] == 1   # its low-S twin verifies
+    for c in golden["edge_cases"]["schnorr"]:
+        m, sg, pk = (bytes.fromhex(c[k]) for k in ("msg", "sig", "pub"))
+        v, R = _textbook_schnorr(m, sg, pk)
+        assert v == c["verdict"], c["name"]
+        if c["name"].startswith("r_infinity"):
+            assert R is None and lift_x(int.from_bytes(pk, "big")) is not None and 0 < int.from_bytes(sg[32:], "big") < N
 
 
 def test_msm_cases(golden):

@@ -170,6 +170,10 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_event_destroy": (None, [vp]),
         "ufsecp_gpu_ext_stream": (vp, [vp]),
         "ufsecp_gpu_ext_synchronize": (c_int, [vp]),
+        "ufsecp_gpu_ext_verify_lanes": (c_uint32, [vp, c_uint32]),
+        "ufsecp_gpu_ext_stream_create": (c_int, [vp, POINTER(vp)]),
+        "ufsecp_gpu_ext_stream_synchronize": (c_int, [vp]),
+        "ufsecp_gpu_ext_stream_destroy": (None, [vp]),
         "ufsecp_gpu_ext_selftest": (c_int, [vp, c_int, vp, vp, vp, c_size_t]),
         "ufsecp_gpu_ext_host_alloc": (c_int, [vp, c_size_t, POINTER(vp)]),
         "ufsecp_gpu_ext_host_free": (c_int, [vp, vp]),
@@ -458,6 +462,16 @@ class UfsecpGpu:
 
     def synchronize(self) -> None:
         self._check("ufsecp_gpu_ext_synchronize", self._lib.ufsecp_gpu_ext_synchronize(self._ctx))
+
+    def new_stream(self) -> int:
+        """A new HIP stream on the context's device (default priority); release
+        it with ``destroy_stream``."""
+        st = c_void_p()
+        self._check("ufsecp_gpu_ext_stream_create", self.lib.ufsecp_gpu_ext_stream_create(self.ctx, ctypes.byref(st)))
+        return st.value
+
+    def destroy_stream(self, stream: int) -> None:
+        self.lib.ufsecp_gpu_ext_stream_destroy(c_void_p(stream))
 
     def stream(self) -> int:
         return self._lib.ufsecp_gpu_ext_stream(self._ctx) or 0

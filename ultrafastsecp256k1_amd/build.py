@@ -36,7 +36,7 @@ HIP_FLAGS = [
 ]
 GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "kernels_pubdata.hip", "kernels_recover.hip",
                "kernels_rlc.hip", "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp", "ufsecp_gpu_abi.cpp",
-               "engine_hook.cpp", "batch_rlc.cpp"]
+               "engine_hook.cpp", "batch_rlc.cpp", "ufsecp_gpu_unsupported.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

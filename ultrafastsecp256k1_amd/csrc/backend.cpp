@@ -495,7 +495,11 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
     if ((e = h_tables_.ensure(desc_bytes + tile_bytes + 64)) != hipSuccess ||
         (e = d_descs_.ensure(desc_bytes)) != hipSuccess || (e = d_tiles_.ensure(tile_bytes)) != hipSuccess)
         return fail(GpuError::Memory, "multi-batch tables", e);
-    // The pinned staging area may still feed an earlier copy on this stream.
+    // The descriptor, tile and word tables (device and pinned staging) may
+    // still feed the previous multi-batch launch, on this stream or another:
+    // order this stream after the previous verify launch, then wait on the
+    // host for both, before either table is rewritten.
+    if ((e = verify_begin(st)) != hipSuccess) return fail(GpuError::Backend, "verify stream order", e);
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(GpuError::Backend, "staging wait", e);
     uint8_t* h = (uint8_t*)h_tables_.ptr;
     std::memcpy(h, batches, desc_bytes);
@@ -518,7 +522,6 @@ GpuError RocmBackend::verify_multi_device(const BatchDesc* batches, size_t nbatc
     a.ntiles_dev = d_words;
     a.counter = d_words + 16;
     a.slots = d_slots_.ptr;
-    if ((e = verify_begin(st)) != hipSuccess) return fail(GpuError::Backend, "verify stream order", e);
     if (!use_graph) {
         if ((e = launch_u32_fill(d_words + 16, 0u, 1, st)) != hipSuccess ||
             (e = launch_verify(2, a, gdesc_, d_mid_, verify_grid_[2], st)) != hipSuccess)

@@ -47,8 +47,9 @@ GpuError MsmEngine::schnorr_batch_check(const uint8_t* msgs, const uint8_t* pubx
     const size_t terms = 2 * cmax + 1;
     // inputs: msgs | pubx | sigs; terms: scalars | points; scratch: partial sums,
     // then 64 bytes of MSM output (status at +48) and the s-range flag (+56)
-    if ((e = rlc_in_.ensure(cmax * 128)) != hipSuccess || (e = rlc_terms_.ensure(terms * 65)) != hipSuccess ||
-        (e = rlc_part_.ensure((size_t)rlc_partial_words((uint32_t)cmax) * 4 + 64)) != hipSuccess)
+    if ((e = order(st)) != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", e);
+    if ((e = grow(rlc_in_, cmax * 128)) != hipSuccess || (e = grow(rlc_terms_, terms * 65)) != hipSuccess ||
+        (e = grow(rlc_part_, (size_t)rlc_partial_words((uint32_t)cmax) * 4 + 64)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(batch check buffers)", e);
     uint8_t* d_in = (uint8_t*)rlc_in_.ptr;
     uint8_t* d_scl = (uint8_t*)rlc_terms_.ptr;
@@ -79,6 +80,7 @@ GpuError MsmEngine::schnorr_batch_check(const uint8_t* msgs, const uint8_t* pubx
         uint32_t srange;
         std::memcpy(&status, host + 48, 4);
         std::memcpy(&srange, host + 56, 4);
+        if (status == (int32_t)GpuError::Internal) return be_->fail(GpuError::Internal, "MSM bucket id out of range");
         // infinity (ERR_ARITH) is the all-valid outcome; a finite sum or a
         // point that does not decompress means some signature is invalid
         if (srange != 0 || status != (int32_t)GpuError::Arith) {

@@ -144,16 +144,28 @@ UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg, bool skip) {
 // contiguous -- buckets of different windows interleave, which nothing
 // downstream depends on -- and c bits (two 8-bit passes at c = 16) replace the
 // log2 of the bucket count (three passes).
+//
+// Bound: window w < W - 1 owns ids [wB, (w + 1)B), the top window [(W - 1)B,
+// (W + 1)B).  Prepared halves are below 2^128, so every digit fits (checked on
+// the CPU by replaying this arithmetic for any input); a digit that would
+// not becomes a skip entry and raises *range_err, which the final kernel
+// reports as an internal error, so no later kernel indexes outside bsum.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W) {
+k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W,
+              u32* __restrict__ range_err) {
     const u32 B = 1u << (c - 1);
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
         const sc s = scl[i];
         u32 carry = 0;
         for (int w = 0; w < W; ++w) {
             const int d = msm_digit(s.v, w, c, W, carry);
-            const u32 b = d != 0 ? (u32)(d < 0 ? -d : d) - 1u : (i & (B - 1u));
-            entries[(size_t)w * n2 + i] = msm_entry((u32)w * B + b, i, d < 0, d == 0);
+            const u32 mag = (u32)(d < 0 ? -d : d);
+            const u32 lim = w == W - 1 ? 2u * B : B;
+            const bool out = mag > lim;
+            if (out) atomicOr(range_err, 1u);
+            const bool skip = d == 0 || out;
+            const u32 b = skip ? (i & (B - 1u)) : mag - 1u;
+            entries[(size_t)w * n2 + i] = msm_entry((u32)w * B + b, i, d < 0, skip);
         }
     }
 }
@@ -421,11 +433,12 @@ __global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ wsum, 
     // their products over three lanes (jac26_dbl_lanes), lane 0 writes
     if (blockIdx.x != 0) return;
     const bool writer = threadIdx.x == 0;
-    const u32 e = *err;
-    if (e != 0xFFFFFFFFu) {
+    const u32 e = err[0];
+    if (e != 0xFFFFFFFFu || err[1] != 0u) {
         if (writer) {
             for (int i = 0; i < 33; ++i) out33[i] = 0;
-            *status = (e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */;
+            *status = e != 0xFFFFFFFFu ? ((e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */)
+                                       : 9 /* INTERNAL: a bucket id out of range */;
         }
         return;
     }
@@ -547,7 +560,35 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
     return p;
 }
 
+hipError_t MsmEngine::order(hipStream_t st) {
+    if (!done_) {
+        hipError_t e = hipEventCreateWithFlags(&done_, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    if (pending_ && last_stream_ != st) return hipStreamWaitEvent(st, done_, 0);
+    return hipSuccess;
+}
+
+hipError_t MsmEngine::grow(DeviceBuffer& b, size_t bytes) {
+    if (bytes > b.cap && pending_) {
+        hipError_t e = hipEventSynchronize(done_);
+        if (e != hipSuccess) return e;
+        pending_ = false;
+    }
+    return b.ensure(bytes);
+}
+
+hipError_t MsmEngine::finish(hipStream_t st) {
+    last_stream_ = st;
+    pending_ = true;
+    return hipEventRecord(done_, st);
+}
+
 MsmEngine::~MsmEngine() {
+    if (done_) {
+        (void)hipEventSynchronize(done_);
+        (void)hipEventDestroy(done_);
+    }
     DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &entries_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
                            &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_count_,
@@ -558,6 +599,17 @@ MsmEngine::~MsmEngine() {
 GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                             int32_t* d_status, int c, hipStream_t st) {
     if (c != 0 && (c < 2 || c > 20)) return be_->fail(GpuError::BadInput, "MSM window width out of range");
+    hipError_t eo = order(st);
+    if (eo != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", eo);
+    const GpuError r = enqueue_work(d_scalars32, d_points33, n, d_out33, d_status, c, st);
+    // whatever was enqueued, later calls on other streams wait for it
+    eo = finish(st);
+    if (r == GpuError::Ok && eo != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", eo);
+    return r;
+}
+
+GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                                 int32_t* d_status, int c, hipStream_t st) {
     const MsmPlan p = msm_plan(n, c, &be_->tuning());
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
@@ -569,15 +621,15 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     const size_t chunks0 = (max_entries + K - 1) / K;
     const size_t slots0 = 2 * chunks0;
     hipError_t e;
-    if ((e = pts_.ensure(2 * n * sizeof(aff))) != hipSuccess || (e = scl_.ensure(2 * n * sizeof(sc))) != hipSuccess ||
-        (e = entries_.ensure(max_entries * 8)) != hipSuccess || (e = sorted_.ensure(max_entries * 8)) != hipSuccess ||
-        (e = bsum_.ensure(nbk * jsz)) != hipSuccess || (e = seg_.ensure(nseg * jsz)) != hipSuccess ||
-        (e = red_a_.ensure(nseg * jsz)) != hipSuccess || (e = red_b_.ensure(nseg * jsz)) != hipSuccess ||
-        (e = err_.ensure(4)) != hipSuccess || (e = slot_keys_[0].ensure(slots0 * 4)) != hipSuccess ||
-        (e = slot_keys_[1].ensure(slots0 * 4)) != hipSuccess || (e = slot_part_[0].ensure(slots0 * jsz)) != hipSuccess ||
-        (e = slot_part_[1].ensure(slots0 * jsz)) != hipSuccess || (e = sel_idx_.ensure(slots0 * 4)) != hipSuccess ||
-        (e = sel_count_.ensure(16)) != hipSuccess ||
-        (e = run_.ensure(slots0 * 12)) != hipSuccess)
+    if ((e = grow(pts_, 2 * n * sizeof(aff))) != hipSuccess || (e = grow(scl_, 2 * n * sizeof(sc))) != hipSuccess ||
+        (e = grow(entries_, max_entries * 8)) != hipSuccess || (e = grow(sorted_, max_entries * 8)) != hipSuccess ||
+        (e = grow(bsum_, nbk * jsz)) != hipSuccess || (e = grow(seg_, nseg * jsz)) != hipSuccess ||
+        (e = grow(red_a_, nseg * jsz)) != hipSuccess || (e = grow(red_b_, nseg * jsz)) != hipSuccess ||
+        (e = grow(err_, 8)) != hipSuccess || (e = grow(slot_keys_[0], slots0 * 4)) != hipSuccess ||
+        (e = grow(slot_keys_[1], slots0 * 4)) != hipSuccess || (e = grow(slot_part_[0], slots0 * jsz)) != hipSuccess ||
+        (e = grow(slot_part_[1], slots0 * jsz)) != hipSuccess || (e = grow(sel_idx_, slots0 * 4)) != hipSuccess ||
+        (e = grow(sel_count_, 16)) != hipSuccess ||
+        (e = grow(run_, slots0 * 12)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
     const unsigned prep_blocks = cus * 4;
@@ -602,10 +654,11 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
     if (max_bytes > tmp_bytes) tmp_bytes = max_bytes;
     if (if_bytes > tmp_bytes) tmp_bytes = if_bytes;
     sel_bytes = max_bytes = if_bytes = tmp_bytes;
-    if ((e = scan_tmp_.ensure(tmp_bytes + 16)) != hipSuccess)
+    if ((e = grow(scan_tmp_, tmp_bytes + 16)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
 
-    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
+    // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
+    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
         (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks, st)) !=
             hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
@@ -618,10 +671,10 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points3
         e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)entries_.ptr,
                                               (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM sort size query", e);
-        if ((e = sort_tmp_.ensure(sort_bytes + 16)) != hipSuccess)
+        if ((e = grow(sort_tmp_, sort_bytes + 16)) != hipSuccess)
             return be_->fail(GpuError::Memory, "hipMalloc(MSM sort temp)", e);
         hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl_.ptr,
-                           n2, (uint64_t*)entries_.ptr, p.c, p.windows);
+                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, err + 1);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
         e = hipcub::DeviceRadixSort::SortKeys(sort_tmp_.ptr, sort_bytes, (const uint64_t*)entries_.ptr,
                                               (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
@@ -707,8 +760,9 @@ GpuError MsmEngine::msm_host(const uint8_t* scalars32, const uint8_t* points33, 
     if (!be_->is_ready()) return be_->fail(GpuError::Device, "context not initialised");
     hipStream_t st = be_->stream();
     hipError_t e;
-    if ((e = in_s_.ensure(n * 32)) != hipSuccess || (e = in_p_.ensure(n * 33)) != hipSuccess ||
-        (e = out_.ensure(64)) != hipSuccess)
+    if ((e = order(st)) != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", e);
+    if ((e = grow(in_s_, n * 32)) != hipSuccess || (e = grow(in_p_, n * 33)) != hipSuccess ||
+        (e = grow(out_, 64)) != hipSuccess)
         return be_->fail(GpuError::Memory, "hipMalloc(MSM inputs)", e);
     e = hipMemcpyAsync(in_s_.ptr, scalars32, n * 32, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(in_p_.ptr, points33, n * 33, hipMemcpyHostToDevice, st);
@@ -724,8 +778,10 @@ GpuError MsmEngine::msm_host(const uint8_t* scalars32, const uint8_t* points33, 
     int32_t status;
     std::memcpy(&status, host + 48, 4);
     if (status != 0) {
-        const char* m = status == 7 ? "MSM result is infinity"
-                        : status == 3 ? "invalid point in array" : "scalar >= n";
+        const char* m = status == 7   ? "MSM result is infinity"
+                        : status == 3 ? "invalid point in array"
+                        : status == 9 ? "MSM bucket id out of range (internal)"
+                                      : "scalar >= n";
         return be_->fail((GpuError)status, m);
     }
     std::memcpy(out33, host, 33);

@@ -77,7 +77,19 @@ public:
 private:
     GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                      int32_t* d_status, int c, hipStream_t st);
+    GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                          int32_t* d_status, int c, hipStream_t st);
+    // Every call shares the scratch buffers below.  order(st) makes st wait
+    // for the previous call when that ran on another stream; grow() waits on
+    // the host for the previous call before a buffer it may still read is
+    // reallocated; finish(st) marks the end of this call's work on st.
+    hipError_t order(hipStream_t st);
+    hipError_t grow(DeviceBuffer& b, size_t bytes);
+    hipError_t finish(hipStream_t st);
     RocmBackend* be_;
+    hipEvent_t done_ = nullptr;
+    hipStream_t last_stream_ = nullptr;
+    bool pending_ = false;
     DeviceBuffer in_s_, in_p_, pts_, scl_, entries_, bsum_, seg_, red_a_, red_b_, err_,
         scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_count_;
     DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_, sorted_, sort_tmp_;

@@ -9,6 +9,7 @@
 #include <limits>
 #include <memory>
 #include <new>
+#include <string>
 
 #include "../../include/ufsecp/ufsecp_gpu.h"
 #include "../../include/ufsecp/ufsecp_gpu_ext.h"
@@ -54,6 +55,15 @@ struct ufsecp_gpu_server {
 #define UFSECP_GPU_CATCH                                           \
     catch (const std::bad_alloc&) { return UFSECP_ERR_GPU_MEMORY; } \
     catch (...) { return UFSECP_ERR_INTERNAL; }
+
+// Records "<op>: not supported by the ROCm backend" as ctx's last error and
+// returns UFSECP_ERR_GPU_UNSUPPORTED (used by ufsecp_gpu_unsupported.cpp).
+ufsecp_error_t ufs_abi_unsupported(ufsecp_gpu_ctx* ctx, const char* op) {
+    std::string m = op;
+    m += ": not supported by the ROCm backend";
+    ctx->backend->fail(GpuError::Unsupported, m.c_str());
+    return UFSECP_ERR_GPU_UNSUPPORTED;
+}
 
 extern "C" {
 
@@ -546,6 +556,33 @@ void ufsecp_gpu_ext_event_destroy(void* event) {
 void* ufsecp_gpu_ext_stream(ufsecp_gpu_ctx* ctx) {
     if (!ctx) return nullptr;
     return (void*)ctx->backend->stream();
+}
+
+uint32_t ufsecp_gpu_ext_verify_lanes(ufsecp_gpu_ctx* ctx, uint32_t kind) {
+    if (!ctx || kind > 2) return 0;
+    const int k = kind == 0 ? 0 : kind == 1 ? 1 : 2;
+    return ctx->backend->verify_grid(k) * (uint32_t)ufs::kVerifyBlock;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_stream_create(ufsecp_gpu_ctx* ctx, void** stream_out) {
+    if (!ctx || !stream_out) return UFSECP_ERR_NULL_ARG;
+    *stream_out = nullptr;
+    if (hipSetDevice(ctx->backend->device()) != hipSuccess) return UFSECP_ERR_GPU_DEVICE;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return UFSECP_ERR_GPU_QUEUE;
+    *stream_out = (void*)st;
+    return UFSECP_OK;
+}
+
+ufsecp_error_t ufsecp_gpu_ext_stream_synchronize(void* stream) {
+    if (!stream) return UFSECP_ERR_NULL_ARG;
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_BACKEND;
+}
+
+void ufsecp_gpu_ext_stream_destroy(void* stream) {
+    if (!stream) return;
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    (void)hipStreamDestroy((hipStream_t)stream);
 }
 
 ufsecp_error_t ufsecp_gpu_ext_synchronize(ufsecp_gpu_ctx* ctx) {

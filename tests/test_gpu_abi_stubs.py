@@ -21,18 +21,18 @@ def _buf(n, fill=0):
 
 
 def _calls(lib, ctx, count, out, inp):
-    """(name, call) for every unsupported op with `count` items, output `out`
-    (cleared by the call) and input buffer `inp`."""
+    """(name, output bytes per item, call) for every unsupported op with
+    `count` items, output `out` (cleared by the call) and input `inp`."""
     c = SZ(count)
     return [
-        ("generator_mul_batch", lambda: lib.ufsecp_gpu_generator_mul_batch(ctx, inp, c, out)),
-        ("ecdh_batch", lambda: lib.ufsecp_gpu_ecdh_batch(ctx, inp, inp, c, out)),
-        ("hash160_pubkey_batch", lambda: lib.ufsecp_gpu_hash160_pubkey_batch(ctx, inp, c, out)),
-        ("tagged_hash", lambda: lib.ufsecp_gpu_tagged_hash(ctx, inp, inp, SZ(32), c, out)),
-        ("hash256", lambda: lib.ufsecp_gpu_hash256(ctx, inp, SZ(64), c, out)),
-        ("frost", lambda: lib.ufsecp_gpu_frost_verify_partial_batch(ctx, inp, inp, inp, inp, inp, inp, inp, inp,
-                                                                    c, out)),
-        ("zk_ecdsa_snark", lambda: lib.ufsecp_gpu_zk_ecdsa_snark_witness_batch(ctx, inp, inp, inp, c, out)),
+        ("generator_mul_batch", 33, lambda: lib.ufsecp_gpu_generator_mul_batch(ctx, inp, c, out)),
+        ("ecdh_batch", 32, lambda: lib.ufsecp_gpu_ecdh_batch(ctx, inp, inp, c, out)),
+        ("hash160_pubkey_batch", 20, lambda: lib.ufsecp_gpu_hash160_pubkey_batch(ctx, inp, c, out)),
+        ("tagged_hash", 32, lambda: lib.ufsecp_gpu_tagged_hash(ctx, inp, inp, SZ(32), c, out)),
+        ("hash256", 32, lambda: lib.ufsecp_gpu_hash256(ctx, inp, SZ(64), c, out)),
+        ("frost", 1, lambda: lib.ufsecp_gpu_frost_verify_partial_batch(ctx, inp, inp, inp, inp, inp, inp, inp, inp,
+                                                                       c, out)),
+        ("zk_ecdsa_snark", 760, lambda: lib.ufsecp_gpu_zk_ecdsa_snark_witness_batch(ctx, inp, inp, inp, c, out)),
     ]
 
 
@@ -40,16 +40,17 @@ def test_unsupported_ops_on_a_live_context(gpu):
     lib, ctx = gpu.lib, gpu.ctx
     inp = _buf(4096, 2)            # every "pubkey" prefix byte 02: passes the prefix checks
     out = _buf(1024, 0xEE)
-    for name, call in _calls(lib, ctx, 1, out, inp):
+    for name, width, call in _calls(lib, ctx, 1, out, inp):
         ctypes.memset(out, 0xEE, 1024)
         assert call() == UFSECP_ERR_GPU_UNSUPPORTED, name
-        assert bytes(out)[:20] == bytes(20), name     # the result buffer was cleared
+        assert bytes(out)[:width] == bytes(width), name       # the item's result was cleared
+        assert bytes(out)[width:] == b"\xee" * (1024 - width), name   # and nothing past it written
         assert lib.ufsecp_gpu_last_error(ctx) == UFSECP_ERR_GPU_UNSUPPORTED
         assert b"not supported" in lib.ufsecp_gpu_last_error_msg(ctx)
-    for name, call in _calls(lib, ctx, 0, None, None):
+    for name, _, call in _calls(lib, ctx, 0, None, None):
         assert call() == UFSECP_OK, name                # count 0: OK, nothing read or written
     big = (1 << 26) + 1
-    for name, call in _calls(lib, ctx, big, None, inp):
+    for name, _, call in _calls(lib, ctx, big, None, inp):
         assert call() == UFSECP_ERR_BAD_INPUT, name
     bad = _buf(4096, 5)            # prefix 05 everywhere
     assert lib.ufsecp_gpu_ecdh_batch(ctx, bad, bad, SZ(2), _buf(64)) == UFSECP_ERR_BAD_PUBKEY

@@ -32,6 +32,15 @@ W_SCHNORR = 119_824
 W_ECDSA = 120_824
 BYTES_SCHNORR = 128 + 1
 BYTES_ECDSA = 129 + 1
+# MSM of N terms at window c: N (floor(256/c) + 1) mixed additions (7M + 4S =
+# 680) plus (floor(256/c) + 1) 2^c bucket-sum additions (12M + 4S = 1,040);
+# decompressing a 33-byte base (sqrt chain + curve check) is 12,228 more.
+W_DECOMPRESS = 12_228
+
+
+def w_msm(n: int, c: int) -> int:
+    windows = 256 // c + 1
+    return n * windows * 680 + windows * (1 << c) * 1040
 
 
 def log(*a):
@@ -146,6 +155,7 @@ def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) 
     batches = [(k, s, pools[k][0].at(32 * f), pools[k][1].at(pstride[k] * f), pools[k][2].at(64 * f),
                 outs[k].at(f)) for k, f, s in layout]
     table = gpu.batch_table(batches)
+    plain_batches = min(plain_batches, nbatches)
     sub = gpu.batch_table(batches[:plain_batches])
     sub_items = sum(s for _, _, s in layout[:plain_batches])
     total = pe.n + ps.n
@@ -170,7 +180,16 @@ def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) 
     ms_sub_persist = timed(lambda: gpu.verify_multi_device(sub, plain_batches, use_graph=True), reps)
     for o in outs:
         o.upload(np.zeros(o.nbytes, np.uint8))
-    ms_sub_plain = timed(lambda: gpu.verify_each_device(sub, plain_batches), 1)
+    # plain form: one launch per batch; warmed on 64 batches, then timed once
+    # over all `plain_batches` (a second pass would double the leg's time)
+    gpu.verify_each_device(gpu.batch_table(batches[:64]), 64)
+    gpu.synchronize()
+    a, b = gpu.event(), gpu.event()
+    a.record()
+    gpu.verify_each_device(sub, plain_batches)
+    b.record()
+    gpu.synchronize()
+    ms_sub_plain = a.elapsed_ms(b)
     got2 = [o.download() for o in outs]
     same = True
     # plain launches covered the first `plain_batches` batches: compare those verdicts
@@ -179,8 +198,10 @@ def bench_mix(gpu, nbatches: int, max_size: int, plain_batches: int, reps: int) 
     return {"batches": nbatches, "items": total, "ecdsa_items": pe.n, "schnorr_items": ps.n,
             "persistent_graph_ms": round(ms_graph, 3), "persistent_direct_ms": round(ms_direct, 3),
             "persistent_sigs_per_s": total / (ms_graph * 1e-3),
-            "subset_batches": plain_batches, "subset_items": sub_items,
-            "subset_persistent_ms": round(ms_sub_persist, 3), "subset_plain_per_batch_ms": round(ms_sub_plain, 3),
+            "compared_batches": plain_batches, "compared_items": sub_items,
+            "compared_all_batches": plain_batches >= nbatches,
+            "compared_persistent_ms": round(ms_sub_persist, 3), "compared_plain_per_batch_ms": round(ms_sub_plain, 3),
+            "plain_sigs_per_s": sub_items / (ms_sub_plain * 1e-3),
             "speedup_persistent_vs_plain": ms_sub_plain / ms_sub_persist,
             "parity": parity, "plain_matches_persistent": same}
 
@@ -240,36 +261,141 @@ def bench_lbtc(gpu, ecdsa_batch, schnorr_batch, reps: int) -> dict:
     return out
 
 
-def bench_msm(gpu, n: int, windows, reps: int) -> dict:
-    """Pippenger MSM over n terms with bases k_i*G: c sweep, device-resident
-    inputs, result checked against the closed form (sum s_i k_i) G."""
+def bench_msm(gpu, n: int, windows, reps: int, mad_peak: float, cpu_threads: int) -> dict:
+    """Pippenger MSM over n terms with bases k_i*G: c sweep with device-resident
+    inputs, (i) 33-byte compressed bases (the ABI path, decompression
+    included) and (ii) the same bases pre-decompressed to 64-byte affine x || y
+    (the bucket phase alone); each result checked against the closed form
+    (sum s_i k_i) G.  Beside it the reference CPU ufsecp_multi_scalar_mul on
+    the host cores over the same terms (its point compared with the GPU's), and
+    the ABI call from host buffers (wall clock, uploads included)."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "tests"))
     from ec_math import compress, mul
     from ultrafastsecp256k1_amd import workload as wl
     t0 = time.perf_counter()
-    sc, pts, ks = wl.generate_msm(n, threads=16)
+    sc, pts, ks, p64 = wl.generate_msm(n, threads=16, affine=True)
     expect = compress(mul(wl.msm_closed_form_scalar(sc, ks)))
     log(f"[bench] msm: {n} terms generated in {time.perf_counter() - t0:.1f}s")
-    d_s, d_p = gpu.to_device(sc), gpu.to_device(pts)
+    d_s, d_p, d_a = gpu.to_device(sc), gpu.to_device(pts), gpu.to_device(p64)
     d_o = gpu.empty(64)
-    res = {}
-    for c in windows:
-        gpu.msm_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c)
-        gpu.synchronize()
-        a, b = gpu.event(), gpu.event()
-        a.record()
-        for _ in range(reps):
-            gpu.msm_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c)
-        b.record()
-        ms = a.elapsed_ms(b) / reps
-        out = d_o.download()
-        status = int(np.frombuffer(out[48:52].tobytes(), np.int32)[0])
-        res[str(c)] = {"ms": round(ms, 3), "ok": status == 0 and out[:33].tobytes() == expect}
-        log(f"[bench] msm n={n} c={c}: {ms:.2f} ms ok={res[str(c)]['ok']}")
+
+    def sweep(launch, label):
+        res = {}
+        for c in windows:
+            launch(c)
+            gpu.synchronize()
+            a, b = gpu.event(), gpu.event()
+            a.record()
+            for _ in range(reps):
+                launch(c)
+            b.record()
+            ms = a.elapsed_ms(b) / reps
+            out = d_o.download()
+            status = int(np.frombuffer(out[48:52].tobytes(), np.int32)[0])
+            res[str(c)] = {"ms": round(ms, 3), "ok": status == 0 and out[:33].tobytes() == expect}
+            log(f"[bench] msm ({label}) n={n} c={c}: {ms:.2f} ms ok={res[str(c)]['ok']}")
+        return res
+
+    res = sweep(lambda c: gpu.msm_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c), "compressed")
+    res_aff = sweep(lambda c: gpu.msm_affine_device(d_s.ptr, d_a.ptr, n, d_o.ptr, d_o.at(48), c), "affine")
     best = min(res, key=lambda c: res[c]["ms"])
-    return {"n": n, "sweep": res, "best_c": int(best), "best_ms": res[best]["ms"],
-            "terms_per_s": n / (res[best]["ms"] * 1e-3), "parity": all(v["ok"] for v in res.values())}
+    best_aff = min(res_aff, key=lambda c: res_aff[c]["ms"])
+    cb, ca = int(best), int(best_aff)
+    work = w_msm(n, cb) + n * W_DECOMPRESS
+    work_aff = w_msm(n, ca)
+    ms, ms_aff = res[best]["ms"], res_aff[best_aff]["ms"]
+    roofline = {"bound": "valu", "unit": "TMAC32/s", "peak": mad_peak / 1e12,
+                "achieved": work / (ms * 1e-3) / 1e12, "frac": work / (ms * 1e-3) / mad_peak,
+                "work_mac32": work, "c": cb,
+                "bucket_phase": {"achieved": work_aff / (ms_aff * 1e-3) / 1e12,
+                                 "frac": work_aff / (ms_aff * 1e-3) / mad_peak, "work_mac32": work_aff, "c": ca,
+                                 "ms": ms_aff},
+                "model": "W_msm(N,c) = N (floor(256/c)+1) 680 + (floor(256/c)+1) 2^c 1040 MAC32, plus N 12228 "
+                         "for decompressing 33-byte bases"}
+    # the ABI call from host buffers: uploads + MSM + 33-byte download
+    best_host = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        got = gpu.msm(sc, pts)
+        best_host = min(best_host, time.perf_counter() - t0)
+    parity = all(v["ok"] for v in res.values()) and all(v["ok"] for v in res_aff.values()) and got == expect
+    cpu = None
+    if cpu_threads:
+        from oracle import oracle as orc
+        if orc.ref_available():
+            rc, out, secs = orc.ref_msm(sc, pts, cpu_threads)
+            cpu = {"value": n / secs, "unit": "terms/s", "cores": cpu_threads, "kind": "reference",
+                   "sample": f"the full {n}-term workload: reference ufsecp_multi_scalar_mul on {cpu_threads} "
+                             f"contiguous chunks (one thread each), partial points added by one more call",
+                   "seconds": round(secs, 3), "matches_gpu": rc == 0 and out == expect}
+            parity &= cpu["matches_gpu"]
+            log(f"[bench] msm cpu baseline: {cpu}")
+    return {"n": n, "sweep": res, "best_c": cb, "best_ms": ms, "terms_per_s": n / (ms * 1e-3),
+            "affine_sweep": res_aff, "affine_best_c": ca, "affine_best_ms": ms_aff,
+            "abi_host_e2e_ms": round(best_host * 1e3, 3), "roofline": roofline, "cpu_baseline": cpu,
+            "parity": parity}
+
+
+def bench_c0(gpu, reps: int, cpu_threads: int) -> dict:
+    """The plain ECDSA configuration: 4,096 signatures (the seeded C0
+    workload, 1% corrupted).  Kernel-only (inputs resident in HBM, HIP events
+    around one launch) and end to end through ufsecp_gpu_ecdsa_verify_batch
+    from host buffers (wall clock, uploads and the verdict download
+    included), median of `reps` passes after 3 warm-ups; the verdict digest is
+    compared with the reference CPU library's (tests/golden/workload_heads.json),
+    and the reference CPU library verifies the same items on 1 and on
+    `cpu_threads` host threads."""
+    import statistics
+    import numpy as np
+    from ultrafastsecp256k1_amd import workload as wl
+    heads = json.loads((ROOT / "tests" / "golden" / "workload_heads.json").read_text())["ecdsa"]
+    n = heads["n"]
+    b = wl.generate("ecdsa", n, heads["seed"])
+    d_m, d_p, d_s = (gpu.to_device(a) for a in (b.msgs, b.pubkeys, b.sigs))
+    d_o = gpu.empty(n)
+    for _ in range(3):
+        gpu.ecdsa_verify_device(d_m.ptr, d_p.ptr, d_s.ptr, n, d_o.ptr)
+    gpu.synchronize()
+    kern = []
+    for _ in range(reps):
+        a, e = gpu.event(), gpu.event()
+        a.record()
+        gpu.ecdsa_verify_device(d_m.ptr, d_p.ptr, d_s.ptr, n, d_o.ptr)
+        e.record()
+        gpu.synchronize()
+        kern.append(a.elapsed_ms(e))
+    v_dev = d_o.download()
+    for _ in range(3):
+        gpu.ecdsa_verify_batch(b.msgs, b.pubkeys, b.sigs)
+    e2e = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        v = gpu.ecdsa_verify_batch(b.msgs, b.pubkeys, b.sigs)
+        e2e.append((time.perf_counter() - t0) * 1e3)
+    digest = hashlib.sha256(v.tobytes()).hexdigest()
+    parity = digest == heads["sha256_verdicts"] and bool(np.array_equal(v, v_dev))
+    k_ms, e_ms = statistics.median(kern), statistics.median(e2e)
+    out = {"n": n, "kernel_ms": round(k_ms, 4), "kernel_sigs_per_s": n / (k_ms * 1e-3),
+           "e2e_ms": round(e_ms, 4), "e2e_sigs_per_s": n / (e_ms * 1e-3), "passes": reps,
+           "verdict_sha256_matches_reference": digest == heads["sha256_verdicts"], "parity": parity}
+    from oracle import oracle as orc
+    if cpu_threads and orc.ref_available():
+        cpu = {}
+        for th in sorted({1, cpu_threads}):
+            orc.ref_verify_batch("ecdsa", b.msgs[:th], b.pubkeys[:th], b.sigs[:th], th)
+            secs = []
+            for _ in range(5):
+                vc, dt = orc.ref_verify_batch("ecdsa", b.msgs, b.pubkeys, b.sigs, th)
+                secs.append(dt)
+            dt = statistics.median(secs)
+            cpu[f"{th}_threads"] = {"value": n / dt, "unit": "sigs/s", "cores": th, "kind": "reference",
+                                    "ms": round(dt * 1e3, 3), "matches_gpu": bool(np.array_equal(vc, v))}
+            parity &= cpu[f"{th}_threads"]["matches_gpu"]
+        out["cpu_baseline"] = cpu
+        out["parity"] = parity
+    log(f"[bench] c0 (4096 ECDSA): {out}")
+    return out
 
 
 def run_verify(gpu, kind: str, n: int, steps: int, warmup: int) -> dict:
@@ -302,10 +428,18 @@ def run_verify(gpu, kind: str, n: int, steps: int, warmup: int) -> dict:
     verdicts = d_o.download()
     corr = batch.corrupted.astype(bool)
     parity = bool(np.all(verdicts[~corr] == 1) and np.all(verdicts[corr] == 0))
+    # bit-exact against the reference CPU verdicts over all n items, when the
+    # committed digests cover this workload (the 2^20 batches)
+    ref_digest = json.loads((ROOT / "tests" / "golden" / "full_size_digests.json").read_text()).get(f"{kind}_2p20")
+    digest_match = None
+    if ref_digest and ref_digest["n"] == n and ref_digest["seed"] == (wl.SEED_ECDSA if kind == "ecdsa" else wl.SEED_SCHNORR):
+        digest_match = hashlib.sha256(verdicts.tobytes()).hexdigest() == ref_digest["sha256_verdicts"]
+        parity = parity and digest_match
     kern = sum(kern_ms) / len(kern_ms)
     res = {"kind": kind, "n": n, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern,
            "kernel_ms_min": min(kern_ms), "sigs_per_s": n / (wall / steps),
-           "kernel_sigs_per_s": n / (kern * 1e-3), "parity": parity}
+           "kernel_sigs_per_s": n / (kern * 1e-3), "parity": parity,
+           "verdicts_match_reference_digest": digest_match}
     log(f"[bench] {kind} n={n}: {res['ms_per_step']:.2f} ms/step, kernel {kern:.2f} ms, "
         f"{res['sigs_per_s'] / 1e6:.2f} M sigs/s, parity {parity}")
     return res, batch, verdicts
@@ -320,14 +454,15 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-items", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="ecdsa,lbtc,mix,msm,serve",
-                    help="extra measurements after the headline: ecdsa, lbtc (libbitcoin rows/columns end to "
-                         "end), mix (uneven batches), msm, serve")
+    ap.add_argument("--legs", default="c0,ecdsa,lbtc,mix,msm,serve",
+                    help="extra measurements after the headline: c0 (4,096 ECDSA, kernel and end to end), ecdsa "
+                         "(2^20), lbtc (libbitcoin rows/columns end to end), mix (uneven batches), msm, serve")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-items", default="4194304,2097152",
                     help="ECDSA,BIP-340 items pre-generated for the serving leg")
     ap.add_argument("--mix-batches", type=int, default=10000)
-    ap.add_argument("--mix-plain-batches", type=int, default=500)
+    ap.add_argument("--mix-plain-batches", type=int, default=10000,
+                    help="batches of the uneven mix also run as one plain launch each (all of them by default)")
     ap.add_argument("--msm-n", type=int, default=1 << 20)
     ap.add_argument("--msm-windows", default="10,11,12,13,14,15,16")
     args = ap.parse_args()
@@ -358,17 +493,24 @@ def main() -> int:
                 "hbm_gbps": n * (BYTES_SCHNORR if kind == "schnorr" else BYTES_ECDSA) / (head["kernel_ms"] * 1e-3) / 1e9,
                 "peak_source": "valu_peak probe (v_mad_u64_u32, measured on this GPU)"}
     cpu = None
-    if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
+    cores = 0 if args.no_cpu else min(16, os.cpu_count() or 1)
+    if cores:
         cpu = cpu_baseline(kind, batch, cores, args.cpu_items, verdicts)
         log(f"[bench] cpu baseline: {cpu}")
     del verdicts
     legs = {}
     want = [x for x in args.legs.split(",") if x]
+    if "c0" in want:
+        legs["c0"] = bench_c0(gpu, 9, cores)
     ecdsa_batch = batch if kind == "ecdsa" else None
     if "ecdsa" in want and kind != "ecdsa":
-        r, ecdsa_batch, _ = run_verify(gpu, "ecdsa", n, max(3, args.steps // 2), 1)
+        r, ecdsa_batch, ev = run_verify(gpu, "ecdsa", n, max(3, args.steps // 2), 1)
         r["valu_frac"] = r["kernel_sigs_per_s"] * W_ECDSA / mad
+        if cores:
+            r["cpu_baseline"] = cpu_baseline("ecdsa", ecdsa_batch, cores, args.cpu_items, ev)
+            log(f"[bench] ecdsa cpu baseline: {r['cpu_baseline']}")
+            r["parity"] = r["parity"] and r["cpu_baseline"].get("matches_gpu", True)
+        del ev
         legs["ecdsa"] = r
     if "lbtc" in want:
         legs["lbtc"] = bench_lbtc(gpu, ecdsa_batch, batch if kind == "schnorr" else None, 2)
@@ -377,11 +519,13 @@ def main() -> int:
         legs["mix"] = bench_mix(gpu, args.mix_batches, 4096, args.mix_plain_batches, 3)
         log(f"[bench] mix: {legs['mix']}")
     if "msm" in want:
-        legs["msm"] = bench_msm(gpu, args.msm_n, [int(c) for c in args.msm_windows.split(",")], 3)
+        legs["msm"] = bench_msm(gpu, args.msm_n, [int(c) for c in args.msm_windows.split(",")], 3, mad, cores)
     if "serve" in want:
         ne, ns = (int(x) for x in args.serve_items.split(","))
         legs["serve"] = bench_serve(args.serve_seconds, ne, ns)
     parity = head["parity"] and all(v.get("parity", True) for v in legs.values())
+    if cpu is not None:
+        parity = parity and cpu.get("matches_gpu", True)
     latency = None
     if "serve" in legs:
         top = legs["serve"]["batched"]["loads"][-1]

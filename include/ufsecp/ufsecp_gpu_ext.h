@@ -84,6 +84,13 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const u
                                                     const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                                                     int32_t* d_status, int window_bits, void* stream);
 
+/* ufsecp_gpu_ext_msm_device over bases already decompressed: d_points64 holds
+ * n * 64 bytes, x || y big-endian affine coordinates.  A point with x or y >=
+ * p or off the curve is BAD_PUBKEY, like an invalid compressed point. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_affine_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
+                                                           const uint8_t* d_points64, size_t n, uint8_t* d_out33,
+                                                           int32_t* d_status, int window_bits, void* stream);
+
 /* Device memory and timing helpers for callers without their own HIP runtime
  * handle (tests, benchmarks, language bindings).  Copies are synchronous with
  * respect to the context stream. */

@@ -201,3 +201,23 @@ def ref_compact_to_opaque(sig64: bytes):
     if rc < 0:
         raise RuntimeError(f"reference driver failed ({rc})")
     return rc, out.raw
+
+
+def ref_msm(scalars: np.ndarray, points: np.ndarray, threads: int):
+    """ufsecp_multi_scalar_mul of the reference CPU library (oracle/_ref) over
+    `threads` contiguous chunks, partial sums added by one more call.
+    Returns (ABI code, 33-byte result, wall seconds)."""
+    os.environ.setdefault("SECP256K1_CACHE_DIR", "/tmp/ufsecp_ref_cache")
+    os.makedirs(os.environ["SECP256K1_CACHE_DIR"], exist_ok=True)
+    L = lib()
+    fn = L.orc_ref_msm
+    fn.restype = ctypes.c_double
+    fn.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_int)]
+    scalars, points = np.ascontiguousarray(scalars), np.ascontiguousarray(points)
+    out = ctypes.create_string_buffer(33)
+    rc = ctypes.c_int(-1)
+    secs = fn(str(REF_LIB_PATH).encode(), _p(scalars), _p(points), scalars.shape[0], out, threads, ctypes.byref(rc))
+    if secs < 0:
+        raise RuntimeError(f"reference driver failed ({secs})")
+    return rc.value, out.raw, secs

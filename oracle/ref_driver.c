@@ -123,3 +123,86 @@ __attribute__((visibility("default"))) int orc_ref_compact_to_opaque(const char*
     if (destroy) destroy(ctx);
     return rc;
 }
+
+typedef int (*msm_fn)(void*, const uint8_t*, const uint8_t*, size_t, uint8_t*);
+
+typedef struct {
+    ctx_create_fn create;
+    ctx_destroy_fn destroy;
+    msm_fn msm;
+    const uint8_t *scalars, *points;
+    size_t lo, hi;
+    uint8_t part[33];
+    int rc;
+} mjob_t;
+
+static void* mjob_run(void* arg) {
+    mjob_t* j = (mjob_t*)arg;
+    void* ctx = 0;
+    if (j->create(&ctx) != 0) {
+        j->rc = -1;
+        return 0;
+    }
+    j->rc = j->msm(ctx, j->scalars + 32 * j->lo, j->points + 33 * j->lo, j->hi - j->lo, j->part);
+    if (j->destroy) j->destroy(ctx);
+    return 0;
+}
+
+/* Multi-scalar multiplication through the reference CPU C ABI
+ * (ufsecp_multi_scalar_mul) on `threads` threads: the n terms are cut into
+ * contiguous chunks, one call per chunk and thread, and the partial sums
+ * (33-byte compressed points; a chunk whose sum is the point at infinity,
+ * ERR_ARITH, is left out) are added by one more call with unit scalars.
+ * *rc_out gets the first failing chunk's code, else the final call's.
+ * Returns wall seconds (< 0 if the library or a symbol is missing). */
+__attribute__((visibility("default"))) double orc_ref_msm(const char* lib_path, const uint8_t* scalars,
+                                                            const uint8_t* points, size_t n, uint8_t* out33,
+                                                            int threads, int* rc_out) {
+    void* h = dlopen(lib_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1.0;
+    ctx_create_fn create = (ctx_create_fn)dlsym(h, "ufsecp_ctx_create");
+    ctx_destroy_fn destroy = (ctx_destroy_fn)dlsym(h, "ufsecp_ctx_destroy");
+    msm_fn msm = (msm_fn)dlsym(h, "ufsecp_multi_scalar_mul");
+    if (!create || !msm) return -2.0;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((size_t)threads > n) threads = n ? (int)n : 1;
+    pthread_t th[256];
+    mjob_t jobs[256];
+    const size_t chunk = (n + (size_t)threads - 1) / (size_t)threads;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < threads; ++t) {
+        mjob_t* j = &jobs[t];
+        j->create = create; j->destroy = destroy; j->msm = msm;
+        j->scalars = scalars; j->points = points;
+        j->lo = (size_t)t * chunk < n ? (size_t)t * chunk : n;
+        j->hi = (size_t)(t + 1) * chunk < n ? (size_t)(t + 1) * chunk : n;
+        j->rc = 0;
+        if (pthread_create(&th[t], 0, mjob_run, j) != 0) {
+            mjob_run(j);
+            th[t] = 0;
+        }
+    }
+    for (int t = 0; t < threads; ++t)
+        if (th[t]) pthread_join(th[t], 0);
+    uint8_t one[32 * 256] = {0}, parts[33 * 256];
+    size_t k = 0;
+    int rc = 0;
+    for (int t = 0; t < threads && rc == 0; ++t) {
+        if (jobs[t].rc == 7) continue;        /* this chunk sums to infinity */
+        if (jobs[t].rc != 0) { rc = jobs[t].rc; break; }
+        one[32 * k + 31] = 1;
+        for (int b = 0; b < 33; ++b) parts[33 * k + b] = jobs[t].part[b];
+        ++k;
+    }
+    if (rc == 0) {
+        void* ctx = 0;
+        if (create(&ctx) != 0) return -3.0;
+        rc = k ? msm(ctx, one, parts, k, out33) : 7;
+        if (destroy) destroy(ctx);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    *rc_out = rc;
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

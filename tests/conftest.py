@@ -31,4 +31,4 @@ def golden():
     import json
     return {name: json.loads((GOLDEN / f"{name}.json").read_text())
             for name in ("workload_heads", "bip340_vectors", "rfc6979_vectors", "edge_cases", "opaque_vectors",
-                         "pubdata_vectors", "recover_vectors")}
+                         "pubdata_vectors", "recover_vectors", "full_size_digests")}

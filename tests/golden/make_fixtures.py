@@ -30,6 +30,8 @@ Runs only in the container that holds the reference sources: it loads
   key-path vectors of compat/libsecp256k1_shim/tests/test_bip341_vectors.cpp,
   rows built as in compat/libbitcoin_bridge/tests/test_lbtc_commitment.cpp,
   and corruptions, with the reference CPU verdicts.
+* ``full_size_digests.json`` -- input and reference-verdict digests of the
+  whole 2^20-signature BIP-340 and ECDSA workloads.
 * ``recover_vectors.json`` -- ECDSA public-key recovery items (every recovery
   id of reference-signed recoverable signatures, high-S twins, out-of-range
   r/s, crafted ids 2/3 items with R.x = r + n) with the reference CPU
@@ -203,6 +205,35 @@ def workload_heads(ref: Ref) -> dict:
     out["msm"] = {"seed": wl.SEED_MSM, "n": n, "sha256_scalars": hashlib.sha256(scalars).hexdigest(),
                   "sha256_points": hashlib.sha256(points).hexdigest(), "result": h(res),
                   "head": [{"s": h(t["s"]), "k": h(t["k"]), "point": h(p)} for t, p in zip(terms[:16], pts[:16])]}
+    return out
+
+
+def full_size_digests(ref: Ref) -> dict:
+    """The 2^20-signature BIP-340 and ECDSA workloads (the benchmark's batches:
+    seeded generator, first item 0): SHA-256 of the inputs and of the reference
+    CPU verdicts over all 2^20 items (ufsecp_schnorr_verify /
+    ufsecp_ecdsa_verify through oracle/ref_driver.c, 8 threads).  The inputs
+    come from the project's C generator, whose signatures are asserted equal to
+    the reference signer's on every 4099th item."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    out = {}
+    n = 1 << 20
+    for kind, seed, gen in (("schnorr", wl.SEED_SCHNORR, schnorr_item), ("ecdsa", wl.SEED_ECDSA, ecdsa_item)):
+        b = wl.generate(kind, n, seed, threads=8)
+        for i in range(0, n, 4099):
+            m, p, sg, c = gen(ref, seed, i)
+            assert (m, p, sg, c) == (bytes(b.msgs[i]), bytes(b.pubkeys[i]), bytes(b.sigs[i]), int(b.corrupted[i])), i
+        v, secs = orc.ref_verify_batch(kind, b.msgs, b.pubkeys, b.sigs, 8)
+        out[f"{kind}_2p20"] = {
+            "seed": seed, "n": n,
+            "sha256_msgs": hashlib.sha256(b.msgs.tobytes()).hexdigest(),
+            "sha256_pubkeys": hashlib.sha256(b.pubkeys.tobytes()).hexdigest(),
+            "sha256_sigs": hashlib.sha256(b.sigs.tobytes()).hexdigest(),
+            "sha256_verdicts": hashlib.sha256(v.tobytes()).hexdigest(),
+            "valid": int(v.sum()), "corrupted": int(b.corrupted.sum()), "reference_signer_sample_stride": 4099,
+        }
+        print(kind, n, "valid", int(v.sum()), f"reference verify {secs:.1f}s")
     return out
 
 
@@ -683,7 +714,8 @@ def main():
     ref = Ref()
     makers = {"bip340_vectors": bip340_vectors, "rfc6979_vectors": rfc6979_vectors, "edge_cases": edge_cases,
               "workload_heads": workload_heads, "opaque_vectors": opaque_vectors,
-              "pubdata_vectors": pubdata_vectors, "recover_vectors": recover_vectors}
+              "pubdata_vectors": pubdata_vectors, "recover_vectors": recover_vectors,
+              "full_size_digests": full_size_digests}
     for name in sys.argv[1:] or list(makers):
         (HERE / f"{name}.json").write_text(json.dumps(makers[name](ref), indent=1))
         print("wrote", HERE / f"{name}.json")

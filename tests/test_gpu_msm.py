@@ -114,3 +114,67 @@ def test_msm_repeated_bases(gpu, c):
         rc, out = gpu.msm_rc(sc, pts, len(scal), c)
         assert rc == UFSECP_OK, name
         assert out == compress(mul(e)), name
+
+
+@pytest.mark.parametrize("c", [11, 13, 15])
+def test_msm_odd_window_widths_at_2p16(gpu, c):
+    """The sweep's odd widths (a partial top window: 128 - c(W - 1) < c bits)
+    at 2^16 terms, against the closed form."""
+    n = 1 << 16
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 200 + c)
+    rc, out = gpu.msm_rc(sc, pts, n, c)
+    assert rc == UFSECP_OK
+    assert out == compress(mul(wl.msm_closed_form_scalar(sc, ks)))
+
+
+def _msm_affine(gpu, sc, p64, c=0):
+    """(status, 33-byte output) of ufsecp_gpu_ext_msm_affine_device."""
+    n = len(sc)
+    d_s, d_p, d_o = gpu.to_device(sc), gpu.to_device(p64), gpu.empty(64)
+    d_o.upload(np.full(64, 0xAB, np.uint8))
+    gpu.msm_affine_device(d_s.ptr, d_p.ptr, n, d_o.ptr, d_o.at(48), c)
+    gpu.synchronize()
+    raw = d_o.download()
+    return int(np.frombuffer(raw[48:52].tobytes(), np.int32)[0]), raw[:33].tobytes()
+
+
+@pytest.mark.parametrize("n,c", [(5000, 0), (1 << 16, 12), (1 << 16, 16)])
+def test_msm_affine_bases_match_compressed(gpu, n, c):
+    """Pre-decompressed bases (64-byte x || y): the same point as the
+    compressed path and the closed form."""
+    sc, pts, ks, p64 = wl.generate_msm(n, seed=wl.SEED_MSM + 300 + c, affine=True)
+    want = compress(mul(wl.msm_closed_form_scalar(sc, ks)))
+    assert _msm_affine(gpu, sc, p64, c) == (0, want)
+    assert gpu.msm_rc(sc, pts, n, c) == (UFSECP_OK, want)
+
+
+def test_msm_affine_bases_invalid(gpu):
+    """x >= p, y >= p, a point off the curve -> BAD_PUBKEY (3); a scalar >= n
+    -> BAD_INPUT (5); the first failing term decides; output zeroed."""
+    from ec_math import P
+    sc, _, _, p64 = wl.generate_msm(300, seed=11, affine=True)
+    for mut in ("x_ge_p", "y_ge_p", "off_curve", "y_neg_ok"):
+        q = p64.copy()
+        x = int.from_bytes(bytes(q[77, :32]), "big")
+        y = int.from_bytes(bytes(q[77, 32:]), "big")
+        if mut == "x_ge_p":
+            q[77, :32] = np.frombuffer((x + P).to_bytes(32, "big") if x + P < 1 << 256 else P.to_bytes(32, "big"),
+                                       np.uint8)
+        elif mut == "y_ge_p":
+            q[77, 32:] = np.frombuffer((y + P).to_bytes(32, "big") if y + P < 1 << 256 else P.to_bytes(32, "big"),
+                                       np.uint8)
+        elif mut == "off_curve":
+            q[77, 32:] = np.frombuffer(((y + 1) % P).to_bytes(32, "big"), np.uint8)
+        else:                       # -P is on the curve: valid, result changes
+            q[77, 32:] = np.frombuffer((P - y).to_bytes(32, "big"), np.uint8)
+        st, out = _msm_affine(gpu, sc, q)
+        if mut == "y_neg_ok":
+            assert st == 0
+        else:
+            assert (st, out) == (3, bytes(33)), mut
+    s2 = sc.copy()
+    s2[200] = 0xFF          # scalar >= n at term 200, bad point at term 77: the earlier term decides
+    q = p64.copy()
+    q[77, 63] ^= 1
+    assert _msm_affine(gpu, s2, q)[0] == 3
+    assert _msm_affine(gpu, s2, p64)[0] == 5

@@ -173,3 +173,44 @@ def test_server_mapped_submission(gpu):
         srv.drain()
     for (kind, m, p, s, exp), out in zip(pinned, outs):
         assert np.array_equal(out, exp)
+
+
+def test_server_poisson_arrivals_serving_shape(gpu):
+    """The serving configuration's shape: Poisson arrivals of requests whose
+    sizes are log-uniform in [1, 8192], 70% ECDSA / 30% BIP-340, 20% of the
+    keys from a 2^16-key pool, a 2^21-entry key cache.  Requests are submitted
+    from one thread at exponential inter-arrival times (mean 0.4 ms, faster
+    than the GPU verifies them, so launch groups overlap and coalesce); every
+    request's verdicts equal the known ones, and every 8th equals the same
+    request verified alone through the plain ABI."""
+    import math
+    import random
+    import time
+    rnd = random.Random(0xC4)
+    nreq = 240
+    sizes = [min(8192, max(1, int(math.exp(rnd.random() * math.log(8193))))) for _ in range(nreq)]
+    kinds = [KIND_ECDSA if rnd.random() < 0.7 else KIND_SCHNORR for _ in range(nreq)]
+    ne = sum(s for s, k in zip(sizes, kinds) if k == KIND_ECDSA)
+    ns = sum(s for s, k in zip(sizes, kinds) if k == KIND_SCHNORR)
+    pools = {KIND_ECDSA: wl.generate_pool("ecdsa", ne, 0xC40, 0xC4F, pool_size=1 << 16, threads=16),
+             KIND_SCHNORR: wl.generate_pool("schnorr", ns, 0xC41, 0xC4F, pool_size=1 << 16, threads=16)}
+    offs = {KIND_ECDSA: 0, KIND_SCHNORR: 0}
+    reqs = []
+    for s, k in zip(sizes, kinds):
+        src, o = pools[k], offs[k]
+        reqs.append((k, src.msgs[o:o + s], src.pubkeys[o:o + s], src.sigs[o:o + s], 1 - src.corrupted[o:o + s]))
+        offs[k] += s
+    outs = [np.full(len(r[1]), 7, np.uint8) for r in reqs]
+    with VerifyServer(gpu, key_cache_log2=21) as srv:
+        for (k, m, p, s, _), out in zip(reqs, outs):
+            srv.submit(k, m, p, s, out)
+            time.sleep(rnd.expovariate(1.0 / 0.4e-3))
+        srv.drain()
+        st = srv.stats()
+    assert st["requests_completed"] == nreq and st["items_completed"] == ne + ns
+    assert st["groups_launched"] < nreq            # requests were coalesced
+    assert st["latency_p99_us"] >= st["latency_p50_us"] > 0
+    for j, ((k, m, p, s, exp), out) in enumerate(zip(reqs, outs)):
+        assert np.array_equal(out, exp), j
+        if j % 8 == 0:
+            assert np.array_equal(out, _plain(gpu, k, m, p, s)), j

@@ -323,3 +323,14 @@ def test_recover_closed_form():
         want = add_pts(mul((-z * ri) % N), mul(s * ri % N, other))
         ok2, key2 = orc.ecdsa_recover(msg, sig, recid ^ 1)
         assert ok2 == 1 and key2 == compress(want)
+
+
+@pytest.mark.parametrize("kind", ["schnorr", "ecdsa"])
+def test_full_size_workload_inputs_pinned(golden, kind):
+    """The generator reproduces the 2^20-item workloads whose reference
+    verdict digests are recorded (inputs hashed, corruption count)."""
+    f = golden["full_size_digests"][f"{kind}_2p20"]
+    b = wl.generate(kind, f["n"], f["seed"], threads=8)
+    for key, arr in (("sha256_msgs", b.msgs), ("sha256_pubkeys", b.pubkeys), ("sha256_sigs", b.sigs)):
+        assert hashlib.sha256(arr.tobytes()).hexdigest() == f[key], key
+    assert int(b.corrupted.sum()) == f["corrupted"]

@@ -160,6 +160,7 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_verify_each_device": (c_int, [vp, vp, c_size_t, vp]),
         "ufsecp_gpu_ext_msm": (c_int, [vp, vp, vp, c_size_t, vp, c_int]),
         "ufsecp_gpu_ext_msm_device": (c_int, [vp, vp, vp, c_size_t, vp, vp, c_int, vp]),
+        "ufsecp_gpu_ext_msm_affine_device": (c_int, [vp, vp, vp, c_size_t, vp, vp, c_int, vp]),
         "ufsecp_gpu_ext_alloc": (c_int, [vp, c_size_t, POINTER(vp)]),
         "ufsecp_gpu_ext_free": (c_int, [vp, vp]),
         "ufsecp_gpu_ext_copy_to_device": (c_int, [vp, vp, vp, c_size_t]),
@@ -396,6 +397,14 @@ class UfsecpGpu:
                                                  c_void_p(d_out33), c_void_p(d_status), window_bits,
                                                  c_void_p(stream))
         self._check("ufsecp_gpu_ext_msm_device", rc)
+
+    def msm_affine_device(self, d_scalars: int, d_points64: int, n: int, d_out33: int, d_status: int,
+                          window_bits: int = 0, stream: Optional[int] = None) -> None:
+        """msm_device over pre-decompressed bases (64-byte big-endian x || y)."""
+        rc = self._lib.ufsecp_gpu_ext_msm_affine_device(self._ctx, c_void_p(d_scalars), c_void_p(d_points64), n,
+                                                        c_void_p(d_out33), c_void_p(d_status), window_bits,
+                                                        c_void_p(stream))
+        self._check("ufsecp_gpu_ext_msm_affine_device", rc)
 
     # -- device-resident variants (pointers are device addresses) --------------
     def schnorr_verify_device(self, d_msgs: int, d_pubx: int, d_sigs: int, count: int, d_out: int,

@@ -70,7 +70,7 @@ GpuError MsmEngine::schnorr_batch_check(const uint8_t* msgs, const uint8_t* pubx
         e = launch_rlc_terms(d_in, d_in + c * 32, d_in + c * 64, (uint32_t)c, seed, be_->bip340_midstate(), scl, pts,
                              d_part, (uint32_t*)(d_tail + 56), st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "batch check term launch", e);
-        const GpuError r = enqueue(scl, pts, 2 * c + 1, d_tail, (int32_t*)(d_tail + 48), 0, st);
+        const GpuError r = enqueue(scl, pts, false, 2 * c + 1, d_tail, (int32_t*)(d_tail + 48), 0, st);
         if (r != GpuError::Ok) return r;
         uint8_t host[64];
         e = hipMemcpyAsync(host, d_tail, 64, hipMemcpyDeviceToHost, st);

@@ -61,8 +61,38 @@ constexpr int kScalarBits = 128;
 // and split it into the two GLV terms 2i: (|k1|, +-P) and 2i+1: (|k2|, +-phi(P)).
 constexpr int kPrepTermsPerThread = 4;
 
+// A base point given as 64 bytes x || y (big-endian affine coordinates):
+// valid when x, y < p and y^2 == x^3 + 7 (two squarings and a
+// multiplication instead of a square root).
+UFS_DEV bool load_affine64(aff& P, const u8* pp) {
+    u32 x[8], y[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const u8* q = pp + 28 - 4 * l;
+        x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
+        q += 32;
+        y[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
+    }
+    const bool lt = u256_lt_p(x) & u256_lt_p(y);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        P.x.v[l] = x[l];
+        P.y.v[l] = y[l];
+    }
+    fe rhs, lhs, seven;
+    fe_sqr(rhs, P.x);
+    fe_mul(rhs, rhs, P.x);
+    fe_set_u32(seven, 7);
+    fe_add(rhs, rhs, seven);
+    fe_sqr(lhs, P.y);
+    return lt & fe_equal(lhs, rhs);
+}
+
+// AFFINE: points are 64-byte x || y (pre-decompressed bases), else 33-byte
+// SEC1 compressed.
+template <bool AFFINE>
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33, u32 n, aff* __restrict__ pts,
+k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points, u32 n, aff* __restrict__ pts,
               sc* __restrict__ scl, u32* __restrict__ err) {
     const size_t step = (size_t)gridDim.x * blockDim.x * kPrepTermsPerThread;
     for (size_t base = (size_t)blockIdx.x * blockDim.x * kPrepTermsPerThread; base < n; base += step) {
@@ -72,17 +102,22 @@ k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points33,
             u32 k[8];
             msm_load_scalar(k, scalars32 + 32ull * i);
             const bool sok = u256_lt_n(k);
-            const u8* pp = points33 + 33ull * i;
-            const u32 prefix = pp[0];
-            u32 x[8];
-#pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const u8* q = pp + 1 + 28 - 4 * l;
-                x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
-            }
             aff P;
-            bool pok = lift_x(P, x, prefix & 1u);
-            pok &= (prefix == 2u) | (prefix == 3u);
+            bool pok;
+            if (AFFINE) {
+                pok = load_affine64(P, points + 64ull * i);
+            } else {
+                const u8* pp = points + 33ull * i;
+                const u32 prefix = pp[0];
+                u32 x[8];
+#pragma unroll
+                for (int l = 0; l < 8; ++l) {
+                    const u8* q = pp + 1 + 28 - 4 * l;
+                    x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
+                }
+                pok = lift_x(P, x, prefix & 1u);
+                pok &= (prefix == 2u) | (prefix == 3u);
+            }
             if (!sok) {
                 atomicMin(err, 2u * (u32)i);
             } else if (!pok) {
@@ -501,12 +536,16 @@ hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* err, unsigned max_blocks, hipStream_t st) {
+hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, bool affine, uint32_t n, void* pts,
+                              void* scl, uint32_t* err, unsigned max_blocks, hipStream_t st) {
     unsigned g = blocks_for(n, (unsigned)kMsmBlock * kPrepTermsPerThread);
     if (g > max_blocks) g = max_blocks;
-    hipLaunchKernelGGL(k_msm_prepare, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points33, n, (aff*)pts, (sc*)scl,
-                       err);
+    if (affine)
+        hipLaunchKernelGGL(k_msm_prepare<true>, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points, n, (aff*)pts,
+                           (sc*)scl, err);
+    else
+        hipLaunchKernelGGL(k_msm_prepare<false>, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points, n, (aff*)pts,
+                           (sc*)scl, err);
     return hipGetLastError();
 }
 
@@ -596,20 +635,20 @@ MsmEngine::~MsmEngine() {
     for (DeviceBuffer* b : all) b->release();
 }
 
-GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
-                            int32_t* d_status, int c, hipStream_t st) {
+GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
+                            uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st) {
     if (c != 0 && (c < 2 || c > 20)) return be_->fail(GpuError::BadInput, "MSM window width out of range");
     hipError_t eo = order(st);
     if (eo != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", eo);
-    const GpuError r = enqueue_work(d_scalars32, d_points33, n, d_out33, d_status, c, st);
+    const GpuError r = enqueue_work(d_scalars32, d_points, affine, n, d_out33, d_status, c, st);
     // whatever was enqueued, later calls on other streams wait for it
     eo = finish(st);
     if (r == GpuError::Ok && eo != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", eo);
     return r;
 }
 
-GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
-                                 int32_t* d_status, int c, hipStream_t st) {
+GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
+                                 uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st) {
     const MsmPlan p = msm_plan(n, c, &be_->tuning());
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
@@ -659,7 +698,8 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
 
     // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
     if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points33, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks, st)) !=
+        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks,
+                                st)) !=
             hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
     // bucket order: unsorted window-major entries, then a stable radix sort
@@ -748,10 +788,10 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
     return GpuError::Ok;
 }
 
-GpuError MsmEngine::msm_device(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
-                               int32_t* d_status, int c, hipStream_t st) {
+GpuError MsmEngine::msm_device(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
+                               uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st) {
     if (!be_->is_ready()) return be_->fail(GpuError::Device, "context not initialised");
-    GpuError r = enqueue(d_scalars32, d_points33, n, d_out33, d_status, c, st ? st : be_->stream());
+    GpuError r = enqueue(d_scalars32, d_points, affine, n, d_out33, d_status, c, st ? st : be_->stream());
     if (r == GpuError::Ok) be_->clear_error();
     return r;
 }
@@ -769,7 +809,7 @@ GpuError MsmEngine::msm_host(const uint8_t* scalars32, const uint8_t* points33, 
     if (e != hipSuccess) return be_->fail(GpuError::Memory, "copy MSM inputs", e);
     uint8_t* d_out = (uint8_t*)out_.ptr;
     int32_t* d_status = (int32_t*)(d_out + 48);
-    GpuError r = enqueue((const uint8_t*)in_s_.ptr, (const uint8_t*)in_p_.ptr, n, d_out, d_status, c, st);
+    GpuError r = enqueue((const uint8_t*)in_s_.ptr, (const uint8_t*)in_p_.ptr, false, n, d_out, d_status, c, st);
     if (r != GpuError::Ok) return r;
     uint8_t host[64];
     e = hipMemcpyAsync(host, d_out, 64, hipMemcpyDeviceToHost, st);

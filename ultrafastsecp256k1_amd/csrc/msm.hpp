@@ -49,8 +49,8 @@ int msm_default_window(size_t n);
 MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 
 // Device launchers (msm.hip)
-hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points33, uint32_t n, void* pts, void* scl,
-                              uint32_t* err, unsigned max_blocks, hipStream_t st);
+hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, bool affine, uint32_t n, void* pts,
+                              void* scl, uint32_t* err, unsigned max_blocks, hipStream_t st);
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
                              hipStream_t st);
@@ -65,8 +65,9 @@ public:
     // Host buffers; synchronous.  c <= 0 picks the default window.
     GpuError msm_host(const uint8_t* scalars32, const uint8_t* points33, size_t n, uint8_t* out33, int c);
     // Device buffers; asynchronous on st.  *d_status receives the ABI code.
-    GpuError msm_device(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
-                        int32_t* d_status, int c, hipStream_t st);
+    // Points are 33-byte compressed, or 64-byte affine x || y when `affine`.
+    GpuError msm_device(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
+                        uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
     // BIP-340 batch check by a random linear combination, one MSM of 2n + 1
     // terms per chunk (host buffers, synchronous): *all_valid = 1 when every
     // signature verifies, 0 otherwise (a wrong 1 has probability about 2^-128
@@ -75,10 +76,10 @@ public:
                                  int* all_valid);
 
 private:
-    GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+    GpuError enqueue(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33,
                      int32_t* d_status, int c, hipStream_t st);
-    GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
-                          int32_t* d_status, int c, hipStream_t st);
+    GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
+                          uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
     // Every call shares the scratch buffers below.  order(st) makes st wait
     // for the previous call when that ran on another stream; grow() waits on
     // the host for the previous call before a buffer it may still read is

@@ -471,7 +471,21 @@ ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_s
     if (n > kMaxGpuBatchN || window_bits < 0 || window_bits == 1 || window_bits > 20) return UFSECP_ERR_BAD_INPUT;
     if (!d_scalars32 || !d_points33 || !d_out33 || !d_status) return UFSECP_ERR_NULL_ARG;
     try {
-        return to_abi(ctx->msm->msm_device(d_scalars32, d_points33, n, d_out33, d_status, window_bits,
+        return to_abi(ctx->msm->msm_device(d_scalars32, d_points33, false, n, d_out33, d_status, window_bits,
+                                           (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+ufsecp_error_t ufsecp_gpu_ext_msm_affine_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
+                                                const uint8_t* d_points64, size_t n, uint8_t* d_out33,
+                                                int32_t* d_status, int window_bits, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (n > kMaxGpuBatchN || window_bits < 0 || window_bits == 1 || window_bits > 20) return UFSECP_ERR_BAD_INPUT;
+    if (!d_scalars32 || !d_points64 || !d_out33 || !d_status) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->msm->msm_device(d_scalars32, d_points64, true, n, d_out33, d_status, window_bits,
                                            (hipStream_t)stream));
     }
     UFSECP_GPU_CATCH

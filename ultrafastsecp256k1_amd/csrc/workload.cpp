@@ -613,7 +613,9 @@ int wl_schnorr(const KeyPool& kp, u64 seed, u64 first, u64 n, u8* msgs, u8* pubs
 
 // MSM terms: scalars n*32 (uniform 256-bit reduced mod n), points n*33 = k_i*G
 // compressed, ks n*32 (the k_i, for the closed form).
-WL_API int ufsecp_wl_msm(u64 seed, u64 first, u64 n, u8* scalars, u8* points, u8* ks, int threads) {
+// points: n x 33 compressed; points64 (optional, may be null): the same
+// bases as n x 64 big-endian affine x || y.
+WL_API int ufsecp_wl_msm(u64 seed, u64 first, u64 n, u8* scalars, u8* points, u8* points64, u8* ks, int threads) {
     if (n == 0) return 0;
     parallel_chunks(n, threads, [&](u64 b, u64 e) {
         const size_t m = e - b;
@@ -635,6 +637,10 @@ WL_API int ufsecp_wl_msm(u64 seed, u64 first, u64 n, u8* scalars, u8* points, u8
             u8* p = points + 33 * (b + j);
             p[0] = (u8)(0x02 | (Ka[j].y.v[0] & 1));
             to_be(Ka[j].x, p + 1);
+            if (points64) {
+                to_be(Ka[j].x, points64 + 64 * (b + j));
+                to_be(Ka[j].y, points64 + 64 * (b + j) + 32);
+            }
         }
     });
     return 0;

@@ -129,7 +129,8 @@ def _lib() -> ctypes.CDLL:
             fn.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                            ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_int]
         lib.ufsecp_wl_msm.restype = ctypes.c_int
-        lib.ufsecp_wl_msm.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, ctypes.c_int]
+        lib.ufsecp_wl_msm.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp,
+                                      ctypes.c_int]
         _WL = lib
     return _WL
 
@@ -189,16 +190,19 @@ def generate_pool(kind: str, n: int, seed: int, pool_seed: int, pool_size: int =
     return VerifyBatch(kind, msgs, pubs, sigs, corr)
 
 
-def generate_msm(n: int, seed: int = SEED_MSM, first: int = 0, threads: int = 0):
-    """(scalars (N,32), points33 (N,33), closed-form k-weighted sum as int mod n)."""
+def generate_msm(n: int, seed: int = SEED_MSM, first: int = 0, threads: int = 0, affine: bool = False):
+    """(scalars (N,32), points33 (N,33), ks (N,32): base i is k_i G).  With
+    affine=True a fourth array holds the same bases as (N,64) big-endian
+    affine x || y."""
     lib = _lib()
     sc = np.zeros((n, 32), np.uint8)
     pts = np.zeros((n, 33), np.uint8)
     ks = np.zeros((n, 32), np.uint8)
-    rc = lib.ufsecp_wl_msm(seed, first, n, _ptr(sc), _ptr(pts), _ptr(ks), threads)
+    p64 = np.zeros((n, 64), np.uint8) if affine else None
+    rc = lib.ufsecp_wl_msm(seed, first, n, _ptr(sc), _ptr(pts), _ptr(p64) if affine else None, _ptr(ks), threads)
     if rc != 0:
         raise RuntimeError(f"MSM workload generation failed ({rc})")
-    return sc, pts, ks
+    return (sc, pts, ks, p64) if affine else (sc, pts, ks)
 
 
 def mix_plan(nbatches: int, max_size: int, seed: int = SEED_MIX, ecdsa_share: float = 0.7):

@@ -1,4 +1,5 @@
 // Pippenger MSM kernels and host engine.  See msm.hpp for the pipeline.
+#include <algorithm>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -647,58 +648,102 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points,
     return r;
 }
 
+// Byte layout of one MSM's device buffers for n terms at plan p; the
+// hipcub temporary-storage sizes are queried for this n (host only, no
+// allocation).  Offsets are 256-byte aligned from a common base.
+GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L) {
+    L = MsmLayout{};
+    L.max_entries = 2 * n * (size_t)p.windows;   // two GLV terms per term
+    if (L.max_entries > 0x7FFFFFFFu)
+        return be_->fail(GpuError::BadInput, "MSM: too many (term, window) pairs for this window width");
+    L.chunks0 = (L.max_entries + kSegChunk - 1) / kSegChunk;
+    L.slots0 = 2 * L.chunks0;
+    const size_t nbk = (size_t)p.groups * p.buckets;
+    const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
+    const size_t jsz = sizeof(jac), S = L.slots0;
+    // hipcub temporary storage: the selects and the max-scan share one area
+    size_t sel_bytes = 0, if_bytes = 0, max_bytes = 0, sort_bytes = 0;
+    uint32_t* nul = nullptr;
+    hipError_t e = hipcub::DeviceSelect::If(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), nul, nul,
+                                            (int)S, PieceStart{nul, nul}, st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceSelect::If(nullptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), nul, nul, (int)S,
+                                     SlotNotEmpty{nul}, st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, nul, nul, hipcub::Max(), (int)S, st);
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                              (int)L.max_entries, 32, 32 + p.c, st);
+    if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM temporary-storage size query", e);
+    L.scan_bytes = std::max(sel_bytes, std::max(if_bytes, max_bytes));
+    L.sort_bytes = sort_bytes;
+    const size_t sizes[MsmLayout::kParts] = {
+        2 * n * sizeof(aff), 2 * n * sizeof(sc), L.max_entries * 8, L.max_entries * 8, nbk * jsz, nseg * jsz,
+        nseg * jsz, nseg * jsz, 8, S * 4, S * 4, S * jsz, S * jsz, S * 4, 16, S * 12, L.scan_bytes + 16,
+        L.sort_bytes + 16};
+    size_t off = 0;
+    for (int i = 0; i < MsmLayout::kParts; ++i) {
+        L.size[i] = sizes[i];
+        L.off[i] = off;
+        off += (sizes[i] + 255) & ~(size_t)255;
+    }
+    L.total = off;
+    return GpuError::Ok;
+}
+
 GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
                                  uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st) {
     const MsmPlan p = msm_plan(n, c, &be_->tuning());
+    MsmLayout L;
+    GpuError r = msm_layout(n, p, st, L);
+    if (r != GpuError::Ok) return r;
+    // the engine's own grow-only buffers, one per part of the layout
+    DeviceBuffer* bufs[MsmLayout::kParts] = {&pts_, &scl_, &entries_, &sorted_, &bsum_, &seg_, &red_a_, &red_b_,
+                                              &err_, &slot_keys_[0], &slot_keys_[1], &slot_part_[0], &slot_part_[1],
+                                              &sel_idx_, &sel_count_, &run_, &scan_tmp_, &sort_tmp_};
+    void* ptr[MsmLayout::kParts];
+    for (int i = 0; i < MsmLayout::kParts; ++i) {
+        hipError_t e = grow(*bufs[i], L.size[i]);
+        if (e != hipSuccess) return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
+        ptr[i] = bufs[i]->ptr;
+    }
+    return run_msm(L, p, ptr, d_scalars32, d_points, affine, n, d_out33, d_status, st);
+}
+
+// The kernel sequence of one MSM over the buffers ptr[] of layout L.  Every
+// buffer a kernel reads is written earlier in the same sequence (the error
+// words and bucket sums by fill kernels), so no buffer needs to start zeroed.
+GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,
+                            const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33, int32_t* d_status,
+                            hipStream_t st) {
+    enum { kPts, kScl, kEntries, kSorted, kBsum, kSeg, kRedA, kRedB, kErr, kKeys0, kKeys1, kPart0, kPart1, kSelIdx,
+           kSelCount, kRun, kScanTmp, kSortTmp };
     const size_t nbk = (size_t)p.groups * p.buckets;
-    const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
-    const size_t jsz = sizeof(jac);
-    const size_t max_entries = 2 * n * (size_t)p.windows;   // two GLV terms per term
-    if (max_entries > 0x7FFFFFFFu)
-        return be_->fail(GpuError::BadInput, "MSM: too many (term, window) pairs for this window width");
     const uint32_t K = kSegChunk;
-    const size_t chunks0 = (max_entries + K - 1) / K;
-    const size_t slots0 = 2 * chunks0;
+    const size_t max_entries = L.max_entries, chunks0 = L.chunks0, slots0 = L.slots0;
     hipError_t e;
-    if ((e = grow(pts_, 2 * n * sizeof(aff))) != hipSuccess || (e = grow(scl_, 2 * n * sizeof(sc))) != hipSuccess ||
-        (e = grow(entries_, max_entries * 8)) != hipSuccess || (e = grow(sorted_, max_entries * 8)) != hipSuccess ||
-        (e = grow(bsum_, nbk * jsz)) != hipSuccess || (e = grow(seg_, nseg * jsz)) != hipSuccess ||
-        (e = grow(red_a_, nseg * jsz)) != hipSuccess || (e = grow(red_b_, nseg * jsz)) != hipSuccess ||
-        (e = grow(err_, 8)) != hipSuccess || (e = grow(slot_keys_[0], slots0 * 4)) != hipSuccess ||
-        (e = grow(slot_keys_[1], slots0 * 4)) != hipSuccess || (e = grow(slot_part_[0], slots0 * jsz)) != hipSuccess ||
-        (e = grow(slot_part_[1], slots0 * jsz)) != hipSuccess || (e = grow(sel_idx_, slots0 * 4)) != hipSuccess ||
-        (e = grow(sel_count_, 16)) != hipSuccess ||
-        (e = grow(run_, slots0 * 12)) != hipSuccess)
-        return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
     const unsigned prep_blocks = cus * 4;
-    uint32_t* err = (uint32_t*)err_.ptr;
-    uint32_t* sel_idx = (uint32_t*)sel_idx_.ptr;
-    uint32_t* sel_count = (uint32_t*)sel_count_.ptr;
-    uint32_t* run_heads = (uint32_t*)run_.ptr;
+    uint32_t* err = (uint32_t*)ptr[kErr];
+    uint32_t* sel_idx = (uint32_t*)ptr[kSelIdx];
+    uint32_t* sel_count = (uint32_t*)ptr[kSelCount];
+    uint32_t* run_heads = (uint32_t*)ptr[kRun];
     uint32_t* run_start = run_heads + slots0;
     uint32_t* pieces = run_start + slots0;
     uint32_t* npieces = sel_count + 1;
-    size_t tmp_bytes = 0, sel_bytes = 0, max_bytes = 0, if_bytes = 0;
-    e = hipcub::DeviceSelect::If(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces, npieces,
-                                 (int)slots0, PieceStart{run_start, sel_count}, st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceSelect::If(nullptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
-                                     sel_count, (int)slots0, SlotNotEmpty{(const uint32_t*)slot_keys_[0].ptr}, st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)slots0,
-                                              st);
-    if (e != hipSuccess) return be_->fail(GpuError::Launch, "scan size query", e);
-    if (sel_bytes > tmp_bytes) tmp_bytes = sel_bytes;
-    if (max_bytes > tmp_bytes) tmp_bytes = max_bytes;
-    if (if_bytes > tmp_bytes) tmp_bytes = if_bytes;
-    sel_bytes = max_bytes = if_bytes = tmp_bytes;
-    if ((e = grow(scan_tmp_, tmp_bytes + 16)) != hipSuccess)
-        return be_->fail(GpuError::Memory, "hipMalloc(scan temp)", e);
+    size_t sel_bytes = L.scan_bytes, max_bytes = L.scan_bytes, if_bytes = L.scan_bytes;
+    void* scan_tmp = ptr[kScanTmp];
+    void* pts = ptr[kPts];
+    void* scl = ptr[kScl];
+    uint64_t* entries = (uint64_t*)ptr[kEntries];
+    uint64_t* sorted = (uint64_t*)ptr[kSorted];
+    jac* bsum = (jac*)ptr[kBsum];
+    uint32_t* slot_keys[2] = {(uint32_t*)ptr[kKeys0], (uint32_t*)ptr[kKeys1]};
+    jac* slot_part[2] = {(jac*)ptr[kPart0], (jac*)ptr[kPart1]};
 
     // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
     if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, pts_.ptr, scl_.ptr, err, prep_blocks,
+        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, pts, scl, err, prep_blocks,
                                 st)) !=
             hipSuccess)
         return be_->fail(GpuError::Launch, "MSM prepare", e);
@@ -707,73 +752,68 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
     {
         const uint32_t n2 = (uint32_t)(2 * n);
         const int bits = p.c;   // bucket bits that differ within one window
-        size_t sort_bytes = 0;
-        e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)entries_.ptr,
-                                              (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
-        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM sort size query", e);
-        if ((e = grow(sort_tmp_, sort_bytes + 16)) != hipSuccess)
-            return be_->fail(GpuError::Memory, "hipMalloc(MSM sort temp)", e);
-        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl_.ptr,
-                           n2, (uint64_t*)entries_.ptr, p.c, p.windows, err + 1);
+        size_t sort_bytes = L.sort_bytes;
+        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl,
+                           n2, entries, p.c, p.windows, err + 1);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
-        e = hipcub::DeviceRadixSort::SortKeys(sort_tmp_.ptr, sort_bytes, (const uint64_t*)entries_.ptr,
-                                              (uint64_t*)sorted_.ptr, (int)max_entries, 32, 32 + bits, st);
+        e = hipcub::DeviceRadixSort::SortKeys(ptr[kSortTmp], sort_bytes, (const uint64_t*)entries,
+                                              sorted, (int)max_entries, 32, 32 + bits, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
     }
     // bucket sums: empty buckets stay at infinity; level 0 over all sorted
     // entries (skips pass through), then chunked reductions of the compacted
     // partials until a single chunk remains.
-    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, (jac*)bsum_.ptr,
+    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, bsum,
                        (uint32_t)nbk);
-    hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts_.ptr,
-                       (const uint64_t*)sorted_.ptr, (uint32_t)max_entries, K, (uint32_t)chunks0,
-                       (jac*)bsum_.ptr, (uint32_t*)slot_keys_[0].ptr, (jac*)slot_part_[0].ptr);
+    hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts,
+                       (const uint64_t*)sorted, (uint32_t)max_entries, K, (uint32_t)chunks0,
+                       bsum, slot_keys[0], slot_part[0]);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
     // run-aligned pieces: most buckets complete here; the pieces of long runs
     // (in slot set 1) continue through the chunked levels below
     {
         const size_t S = slots0;
-        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
-                                     sel_count, (int)S, SlotNotEmpty{(const uint32_t*)slot_keys_[0].ptr}, st);
+        e = hipcub::DeviceSelect::If(scan_tmp, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
+                                     sel_count, (int)S, SlotNotEmpty{(const uint32_t*)slot_keys[0]}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
         hipLaunchKernelGGL(k_run_heads, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,
+                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys[0],
                            (uint32_t)S, run_heads);
-        e = hipcub::DeviceScan::InclusiveScan(scan_tmp_.ptr, max_bytes, run_heads, run_start, hipcub::Max(), (int)S,
+        e = hipcub::DeviceScan::InclusiveScan(scan_tmp, max_bytes, run_heads, run_start, hipcub::Max(), (int)S,
                                               st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM run starts", e);
-        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces,
+        e = hipcub::DeviceSelect::If(scan_tmp, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces,
                                      npieces, (int)S, PieceStart{run_start, sel_count}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM piece list", e);
         hipLaunchKernelGGL(k_piece_reduce, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys_[0].ptr,
-                           (const jac*)slot_part_[0].ptr, (const uint32_t*)pieces, (const uint32_t*)npieces,
-                           (const uint32_t*)run_start, (uint32_t)S, (jac*)bsum_.ptr, (uint32_t*)slot_keys_[1].ptr,
-                           (jac*)slot_part_[1].ptr);
+                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys[0],
+                           (const jac*)slot_part[0], (const uint32_t*)pieces, (const uint32_t*)npieces,
+                           (const uint32_t*)run_start, (uint32_t)S, bsum, slot_keys[1],
+                           slot_part[1]);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM pieces", e);
     }
     size_t slots = slots0;
     int lv = 1;
     for (;;) {
         const size_t chunks = (slots + K - 1) / K;
-        e = hipcub::DeviceSelect::If(scan_tmp_.ptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
-                                     sel_count, (int)slots, SlotNotEmpty{(const uint32_t*)slot_keys_[lv].ptr}, st);
+        e = hipcub::DeviceSelect::If(scan_tmp, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
+                                     sel_count, (int)slots, SlotNotEmpty{(const uint32_t*)slot_keys[lv]}, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
         hipLaunchKernelGGL(k_seg, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
                            (const uint32_t*)sel_idx, (const uint32_t*)sel_count,
-                           (const uint32_t*)slot_keys_[lv].ptr, (const jac*)slot_part_[lv].ptr, K, (uint32_t)chunks,
-                           (jac*)bsum_.ptr, (uint32_t*)slot_keys_[lv ^ 1].ptr, (jac*)slot_part_[lv ^ 1].ptr);
+                           (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], K, (uint32_t)chunks,
+                           bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
         lv ^= 1;
         if (chunks == 1) break;   // one chunk held every remaining partial: all buckets complete
         slots = 2 * chunks;
     }
-    if ((e = launch_msm_segments(bsum_.ptr, seg_.ptr, p, st)) != hipSuccess)
+    if ((e = launch_msm_segments(bsum, ptr[kSeg], p, st)) != hipSuccess)
         return be_->fail(GpuError::Launch, "MSM window sums", e);
     // tree-reduce segments per window until one point per window remains
     uint32_t len = (uint32_t)(p.buckets / p.seg_len);
-    const void* cur = seg_.ptr;
-    void* bufs[2] = {red_a_.ptr, red_b_.ptr};
+    const void* cur = ptr[kSeg];
+    void* bufs[2] = {ptr[kRedA], ptr[kRedB]};
     int which = 0;
     while (len > 1) {
         const uint32_t out_len = (len + kReduceBlock - 1) / kReduceBlock;

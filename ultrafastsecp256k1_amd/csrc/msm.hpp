@@ -58,6 +58,14 @@ hipError_t launch_msm_final(const void* wsum, const MsmPlan& p, const uint32_t* 
                             int32_t* status, hipStream_t st);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 
+// Device buffers of one MSM, as parts of one byte range: offsets are
+// 256-byte aligned; total is the range's size.
+struct MsmLayout {
+    static constexpr int kParts = 18;
+    size_t off[kParts], size[kParts];
+    size_t total, max_entries, chunks0, slots0, scan_bytes, sort_bytes;
+};
+
 class MsmEngine {
 public:
     explicit MsmEngine(RocmBackend* be) : be_(be) {}
@@ -80,6 +88,10 @@ private:
                      int32_t* d_status, int c, hipStream_t st);
     GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
                           uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
+    GpuError msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L);
+    GpuError run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,
+                     const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33, int32_t* d_status,
+                     hipStream_t st);
     // Every call shares the scratch buffers below.  order(st) makes st wait
     // for the previous call when that ran on another stream; grow() waits on
     // the host for the previous call before a buffer it may still read is

@@ -84,6 +84,25 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const u
                                                     const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                                                     int32_t* d_status, int window_bits, void* stream);
 
+/* ufsecp_gpu_ext_msm_device that allocates nothing: no hipMalloc, hipFree or
+ * allocating library routine on any call.  No counterpart in the reference;
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The MSM's
+ * buffers come from a static 32 MiB device array that every GPU the library
+ * runs on holds until the process ends; n <= 16384 terms, and the window
+ * (window_bits, 0 = tuned) must fit that array: up to 13 bits at 16384
+ * terms.  Larger requests -> BAD_INPUT with nothing enqueued.  Same inputs,
+ * outputs and status codes as ufsecp_gpu_ext_msm_device.  Calls through one
+ * context are ordered; calls on one GPU from different contexts share the
+ * array and race if they overlap. */
+UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device_noalloc(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
+                                                            const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                                                            int32_t* d_status, int window_bits, void* stream);
+
+/* Device and pinned buffers the library has allocated (grown) since it was
+ * loaded: a test hook for allocation-free entry points. */
+UFSECP_API uint64_t ufsecp_gpu_ext_buffer_alloc_count(void);
+
 /* ufsecp_gpu_ext_msm_device over bases already decompressed: d_points64 holds
  * n * 64 bytes, x || y big-endian affine coordinates.  A point with x or y >=
  * p or off the curve is BAD_PUBKEY, like an invalid compressed point. */

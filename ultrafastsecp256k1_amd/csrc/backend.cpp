@@ -1,17 +1,24 @@
 // RocmBackend implementation (host side).  See backend.hpp.
 #include "backend.hpp"
 
+#include <atomic>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
 namespace ufs {
 
+static std::atomic<uint64_t> g_buffer_allocs{0};
+
+uint64_t buffer_alloc_count() { return g_buffer_allocs.load(std::memory_order_relaxed); }
+
 hipError_t DeviceBuffer::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     release();
     size_t want = 1;
     while (want < bytes) want <<= 1;   // grow-only, power of two
+    g_buffer_allocs.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipMalloc(&ptr, want);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -33,6 +40,7 @@ hipError_t PinnedBuffer::ensure(size_t bytes) {
     release();
     size_t want = 1;
     while (want < bytes) want <<= 1;
+    g_buffer_allocs.fetch_add(1, std::memory_order_relaxed);
     hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         ptr = nullptr;

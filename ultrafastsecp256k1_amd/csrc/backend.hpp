@@ -37,6 +37,10 @@ enum class GpuError : int {
     Queue = 106,
 };
 
+// Allocations made by DeviceBuffer / PinnedBuffer growth since the library
+// was loaded (device and pinned host memory).
+uint64_t buffer_alloc_count();
+
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t cap = 0;

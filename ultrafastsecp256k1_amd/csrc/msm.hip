@@ -1,6 +1,7 @@
 // Pippenger MSM kernels and host engine.  See msm.hpp for the pipeline.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include <hipcub/hipcub.hpp>
 
@@ -598,6 +599,80 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
     p.buckets = 1 << (c - 1);
     p.seg_len = p.buckets < 8 ? p.buckets : 8;
     return p;
+}
+
+// ---------------------------------------------------------------------------
+// MSM without allocation: the buffers of one MSM of at most
+// kNoAllocMaxTerms terms are carved from `scratch`, device memory that every
+// GPU the library runs on holds from module load to process end.  In host
+// code `scratch` names a host-side stand-in, never a device address: the
+// device address comes from hipGetSymbolAddress, once per device at context
+// creation (msm_noalloc_setup), and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocMaxTerms = size_t(1) << 14;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};   // device address of `scratch`, per device
+}  // namespace
+
+hipError_t msm_noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_dev[device] = p;
+    return hipSuccess;
+}
+
+static void* msm_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
+GpuError MsmEngine::init() {
+    hipError_t e = hipSetDevice(be_->device());
+    if (e == hipSuccess && !done_) e = hipEventCreateWithFlags(&done_, hipEventDisableTiming);
+    if (e != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", e);
+    if ((e = msm_noalloc_setup(be_->device())) != hipSuccess)
+        return be_->fail(GpuError::Device, "MSM scratch symbol address", e);
+    return GpuError::Ok;
+}
+
+GpuError MsmEngine::msm_device_noalloc(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n,
+                                       uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st) {
+    if (!be_->is_ready()) return be_->fail(GpuError::Device, "context not initialised");
+    if (n > kNoAllocMaxTerms) return be_->fail(GpuError::BadInput, "no-allocation MSM: more than 16384 terms");
+    if (c != 0 && (c < 2 || c > 20)) return be_->fail(GpuError::BadInput, "MSM window width out of range");
+    uint8_t* base = (uint8_t*)msm_noalloc_scratch(be_->device());
+    if (!base || !done_) return be_->fail(GpuError::Device, "no-allocation MSM scratch not set up on this device");
+    st = st ? st : be_->stream();
+    const MsmPlan p = msm_plan(n, c, &be_->tuning());
+    MsmLayout L;
+    GpuError r = msm_layout(n, p, st, L);
+    if (r != GpuError::Ok) return r;
+    if (L.total > kNoAllocScratchBytes)
+        return be_->fail(GpuError::BadInput,
+                         "no-allocation MSM: this term count and window width need more than the 32 MiB scratch "
+                         "(up to 16384 terms with windows of at most 13 bits fit)");
+    void* ptr[MsmLayout::kParts];
+    for (int i = 0; i < MsmLayout::kParts; ++i) ptr[i] = base + L.off[i];
+    hipError_t e = order(st);
+    if (e != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", e);
+    r = run_msm(L, p, ptr, d_scalars32, d_points33, false, n, d_out33, d_status, st);
+    e = finish(st);
+    if (r == GpuError::Ok && e != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", e);
+    if (r == GpuError::Ok) be_->clear_error();
+    return r;
 }
 
 hipError_t MsmEngine::order(hipStream_t st) {

@@ -70,12 +70,21 @@ class MsmEngine {
 public:
     explicit MsmEngine(RocmBackend* be) : be_(be) {}
     ~MsmEngine();
+    // Context set-up (the backend's device current): the completion event and
+    // the device address of the no-allocation scratch.
+    GpuError init();
     // Host buffers; synchronous.  c <= 0 picks the default window.
     GpuError msm_host(const uint8_t* scalars32, const uint8_t* points33, size_t n, uint8_t* out33, int c);
     // Device buffers; asynchronous on st.  *d_status receives the ABI code.
     // Points are 33-byte compressed, or 64-byte affine x || y when `affine`.
     GpuError msm_device(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
                         uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
+    // msm_device without any allocation: buffers carved from the library's
+    // static per-device scratch (32 MiB), n <= 16384, 33-byte points.
+    // Calls through this engine are ordered; calls on one GPU from other
+    // contexts share the same scratch and must not overlap.
+    GpuError msm_device_noalloc(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                                int32_t* d_status, int c, hipStream_t st);
     // BIP-340 batch check by a random linear combination, one MSM of 2n + 1
     // terms per chunk (host buffers, synchronous): *all_valid = 1 when every
     // signature verifies, 0 otherwise (a wrong 1 has probability about 2^-128

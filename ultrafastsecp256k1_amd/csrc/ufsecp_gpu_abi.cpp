@@ -123,7 +123,13 @@ ufsecp_error_t ufsecp_gpu_ctx_create(ufsecp_gpu_ctx** ctx_out, uint32_t backend_
         if (e != GpuError::Ok) return to_abi(e);
         auto* ctx = new (std::nothrow) ufsecp_gpu_ctx;
         if (!ctx) return UFSECP_ERR_INTERNAL;
-        ctx->msm = std::make_unique<ufs::MsmEngine>(backend.get());
+        auto msm = std::make_unique<ufs::MsmEngine>(backend.get());
+        if (msm->init() != GpuError::Ok) {
+            const ufsecp_error_t r = to_abi(backend->last_error());
+            delete ctx;
+            return r;
+        }
+        ctx->msm = std::move(msm);
         ctx->backend = std::move(backend);
         ctx->backend_id = backend_id;
         ctx->device_index = device_index;
@@ -476,6 +482,22 @@ ufsecp_error_t ufsecp_gpu_ext_msm_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_s
     }
     UFSECP_GPU_CATCH
 }
+
+ufsecp_error_t ufsecp_gpu_ext_msm_device_noalloc(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
+                                                 const uint8_t* d_points33, size_t n, uint8_t* d_out33,
+                                                 int32_t* d_status, int window_bits, void* stream) {
+    if (!ctx) return UFSECP_ERR_NULL_ARG;
+    if (n == 0) return UFSECP_OK;
+    if (window_bits < 0 || window_bits == 1 || window_bits > 20) return UFSECP_ERR_BAD_INPUT;
+    if (!d_scalars32 || !d_points33 || !d_out33 || !d_status) return UFSECP_ERR_NULL_ARG;
+    try {
+        return to_abi(ctx->msm->msm_device_noalloc(d_scalars32, d_points33, n, d_out33, d_status, window_bits,
+                                                   (hipStream_t)stream));
+    }
+    UFSECP_GPU_CATCH
+}
+
+uint64_t ufsecp_gpu_ext_buffer_alloc_count(void) { return ufs::buffer_alloc_count(); }
 
 ufsecp_error_t ufsecp_gpu_ext_msm_affine_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
                                                 const uint8_t* d_points64, size_t n, uint8_t* d_out33,

@@ -93,8 +93,39 @@ UFS_DEV void load_le32_bytes(u32 r[8], const u8* p) {
 // 128-140), and high-S is accepted: the reference normalises s to n - s,
 // which leaves x(R) unchanged.  Items are d.stride bytes apart when the
 // stride is set (rows), else packed.
+UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+                                 bool opaque, const sc& w_in);
+
+// s of item i as the inversion sees it: the signature's s when it is in range
+// (strict [1, n), and low-S unless opaque), else 1 -- the item's verdict is
+// false then, and 1 keeps a shared product of s values invertible.
+UFS_DEV void ecdsa_item_s(const BatchDesc& d, u32 i, bool opaque, sc& s) {
+    u32 raw[8];
+    if (opaque) {
+        const u64 st = d.stride;
+        load_le32_bytes(raw, d.sigs + (st ? st : 64ull) * i + 32);
+    } else {
+        load_be32_aligned(raw, d.sigs + 64ull * i + 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s.v[k] = raw[k];
+    bool ok = !u256_is_zero(raw) && u256_lt_n(raw);
+    if (!opaque) ok &= !sc_is_high(s);
+    if (!ok) sc_set_u32(s, 1);
+}
+
 UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
                                bool opaque) {
+    sc s, w;
+    ecdsa_item_s(d, i, opaque, s);
+    sc_inv(w, s);
+    return ecdsa_verify_item_w(d, i, G, slot, pts, opaque, w);
+}
+
+// ECDSA verification of item i of batch d given w = 1/s mod n (computed by
+// the caller, possibly shared with another item).
+UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+                                 bool opaque, const sc& w_in) {
     u32 z_raw[8], r_raw[8], s_raw[8], x_raw[8];
     const u8* pk;
     if (opaque) {
@@ -128,12 +159,11 @@ UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* s
     if (!ok) {
         // keep the lane on the common path with harmless values
         set_generator(P);
-        sc_set_u32(s, 1);
         sc_set_u32(r, 1);
     }
     sc_reduce_once(z);
-    sc w, u1, u2;
-    sc_inv(w, s);
+    sc u1, u2;
+    const sc w = w_in;
     sc_mul(u1, z, w);
     sc_mul(u2, r, w);
     sc k1, k2;
@@ -144,6 +174,50 @@ UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* s
     ecmult_dual(R, slot, k1, n1, k2, n2, G, u1);
     const bool v = !R.inf && ecdsa_xcheck(R, r.v);
     return ok && v;
+}
+
+UFS_DEV void store_sc(fe* p, const sc& a) {
+    fe t;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t.v[k] = a.v[k];
+    store_fe(p, t);
+}
+
+UFS_DEV void load_sc(sc& a, const fe* p) {
+    fe t;
+    load_fe(t, p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[k] = t.v[k];
+}
+
+// ECDSA verification of item i1 and, when `two`, item i2 of batch d, the two
+// sharing the inversion of s (Montgomery's trick: 1/s1 = s2 / (s1 s2),
+// 1/s2 = s1 / (s1 s2)).  Both inverses wait in the slot's last entry while
+// the items run one after the other in a rolled loop (one inline copy of the
+// scalar multiplication).
+UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, const gcomb& G, aff* slot,
+                                const aff* pts, bool& v1, bool& v2) {
+    fe* stash = &slot[kSlotEntries - 1].x;
+    {
+        sc s1, s2, ss, inv, w;
+        ecdsa_item_s(d, i1, false, s1);
+        if (two) ecdsa_item_s(d, i2, false, s2); else sc_set_u32(s2, 1);
+        sc_mul(ss, s1, s2);
+        sc_inv(inv, ss);
+        sc_mul(w, inv, s2);
+        store_sc(stash, w);
+        sc_mul(w, inv, s1);
+        store_sc(stash + 1, w);
+    }
+    v2 = false;
+    const int m = two ? 2 : 1;
+#pragma unroll 1
+    for (int k = 0; k < m; ++k) {
+        sc w;
+        load_sc(w, stash + k);
+        const bool v = ecdsa_verify_item_w(d, k ? i2 : i1, G, slot, pts, false, w);
+        if (k == 0) v1 = v; else v2 = v;
+    }
 }
 
 // BIP-340 verification of item i of batch d up to the y-parity test: R =
@@ -290,9 +364,9 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
             first = t * kTileItems;
             pts = reinterpret_cast<const aff*>(a.single_pts);
         }
-        // single BIP-340 batch: the lane's item of tile t + W joins the one
-        // of tile t (shared inversion); the tile loop then steps by 2W
-        const bool pairing = KIND == 1 && !list;
+        // single ECDSA or BIP-340 batch: the lane's item of tile t + W joins
+        // the one of tile t (shared inversion); the tile loop then steps by 2W
+        const bool pairing = (KIND == 0 || KIND == 1) && !list;
         const u32 t2 = t + nwaves;
         t += pairing ? 2 * nwaves : nwaves;
         const u32 i = first + lane;
@@ -303,6 +377,8 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         bool v, v2 = false;
         if (kind == kKindSchnorr) {
             schnorr_verify_items(d, i, i2, two, mid, G, slot, pts, v, v2);
+        } else if (KIND == 0) {
+            ecdsa_verify_items(d, i, i2, two, G, slot, pts, v, v2);
         } else {
             v = ecdsa_verify_item(d, i, G, slot, pts, kind == kKindEcdsaOpaque);
         }

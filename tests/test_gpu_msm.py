@@ -178,3 +178,20 @@ def test_msm_affine_bases_invalid(gpu):
     q[77, 63] ^= 1
     assert _msm_affine(gpu, s2, q)[0] == 3
     assert _msm_affine(gpu, s2, p64)[0] == 5
+
+
+@pytest.mark.parametrize("n,c", [(1 << 16, 16), (1 << 16, 8), (200_000, 13)])
+def test_msm_one_scalar_many_bases(gpu, n, c):
+    """Every term with the same scalar (and a second run with a quarter of the
+    terms sharing one scalar): whole windows fall into one or two buckets, so
+    the bucket partials survive the piece step and go through several
+    reduction rounds.  Against the closed form s * sum k_i (+ the rest)."""
+    N = wl.N_ORDER
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 400 + c, threads=16)
+    s = int.from_bytes(bytes(sc[0]), "big")
+    same = np.broadcast_to(sc[0], sc.shape).copy()
+    assert gpu.msm_rc(same, pts, n, c) == (UFSECP_OK, compress(mul(s * wl.msm_closed_form_scalar(
+        np.broadcast_to(np.frombuffer((1).to_bytes(32, "big"), np.uint8), sc.shape), ks) % N)))
+    mixed = sc.copy()
+    mixed[::4] = sc[0]
+    assert gpu.msm_rc(mixed, pts, n, c) == (UFSECP_OK, compress(mul(wl.msm_closed_form_scalar(mixed, ks))))

@@ -101,8 +101,31 @@ def test_server_second_pass_hits_cache(gpu):
     for a, b, r in zip(first, second, reqs):
         assert np.array_equal(a, b)
         assert np.array_equal(a, r[4])
-    # nearly every key was inserted on the first pass (direct-mapped table)
+    # nearly every key was inserted on the first pass
     assert st["key_cache_hits"] > 0.8 * st["key_lookups"]
+
+
+def test_server_cache_keeps_reused_keys_under_one_shot_keys(gpu):
+    """Keys found in the cache after their insertion sit in a way that new keys
+    never take: 16,384 keys seen once (eight per set of a 2^12-entry table)
+    leave a warmed pool of 256 keys resident.  A direct-mapped table would keep
+    about e^-4 = 2% of them; here only pool keys that share a set with
+    another pool key can be lost."""
+    pool = wl.generate_pool("ecdsa", 2048, 0x3A, 0x3B, pool_size=256, pool_permille=1000)
+    fresh = wl.generate("ecdsa", 1 << 14, 0x3C)
+    req = (KIND_ECDSA, pool.msgs, pool.pubkeys, pool.sigs, 1 - pool.corrupted)
+    with VerifyServer(gpu, key_cache_log2=12) as srv:
+        for _ in range(3):   # insert, then find (and copy to the kept way)
+            assert np.array_equal(_submit_all(srv, [req], threads=1)[0], req[4])
+        flood = [(KIND_ECDSA, fresh.msgs[j:j + 4096], fresh.pubkeys[j:j + 4096], fresh.sigs[j:j + 4096],
+                  1 - fresh.corrupted[j:j + 4096]) for j in range(0, fresh.n, 4096)]
+        for r, out in zip(flood, _submit_all(srv, flood, threads=1)):
+            assert np.array_equal(out, r[4])
+        srv.stats(reset=True)
+        assert np.array_equal(_submit_all(srv, [req], threads=1)[0], req[4])
+        st = srv.stats()
+    assert st["key_lookups"] == 2048
+    assert st["key_cache_hits"] > 0.85 * st["key_lookups"]
 
 
 def test_server_without_cache(gpu):

@@ -119,8 +119,10 @@ int verify_blocks_per_cu(int kind);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 
 // Public-key decompression through a device-resident cache (serving path).
-// The cache is a direct-mapped table of (x, y with even parity) entries,
-// 2^k entries of 64 bytes; an entry is trusted only if its x matches and
+// The cache is a two-way set-associative table of (x, y with even parity)
+// entries, 2^k entries of 64 bytes in 2^(k-1) sets: new keys go to way 1, a
+// key found in way 1 is copied to way 0, so keys seen once never displace
+// reused ones.  An entry is trusted only if its x matches and
 // y^2 == x^3 + 7, so concurrent writers can never produce a wrong point.
 struct KeyCacheDesc {
     void* entries;

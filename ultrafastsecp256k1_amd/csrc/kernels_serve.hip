@@ -82,14 +82,22 @@ k_key_resolve(const BatchDesc* __restrict__ descs, const TileDesc* __restrict__ 
     }
     bool hit = false;
     if (cache.entries) {
-        const aff* e = reinterpret_cast<const aff*>(cache.entries) + key_hash(x, cache.mask);
-        aff c;
-        load_aff(c, e);
-        bool same = true;
+        // Two ways per set: way 0 holds keys that were found at least once
+        // after their insertion, way 1 takes every new key.  A key found in
+        // way 1 is copied into way 0, so a stream of keys seen once only
+        // replaces way 1 and leaves the reused keys in place.
+        aff* set = reinterpret_cast<aff*>(cache.entries) + 2u * key_hash(x, cache.mask >> 1);
+#pragma unroll 1
+        for (u32 way = 0; way < 2 && !hit; ++way) {
+            aff c;
+            load_aff(c, set + way);
+            bool same = true;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) same &= c.x.v[k] == x[k];
-        if (same) {
-            // trust the entry only if (x, y) is on the curve
+            for (int k = 0; k < 8; ++k) same &= c.x.v[k] == x[k];
+            if (!same) continue;
+            // trust the entry only if (x, y) is on the curve: a torn entry
+            // (another lane's copy or insert in flight) fails the test and is
+            // taken as a miss
             fe x3, y2, seven;
             fe_sqr(x3, c.x);
             fe_mul(x3, x3, c.x);
@@ -101,6 +109,7 @@ k_key_resolve(const BatchDesc* __restrict__ descs, const TileDesc* __restrict__ 
                 P.x = c.x;
                 apply_parity(P.y, c.y, parity);
                 store_aff(pts + i, P);
+                if (way == 1) store_aff(set, c);
                 hit = true;
             }
         }
@@ -131,8 +140,8 @@ k_key_decompress(const BatchDesc* __restrict__ descs, const void* const* __restr
             continue;
         }
         if (cache.entries) {
-            aff* c = reinterpret_cast<aff*>(cache.entries) + key_hash(x, cache.mask);
-            store_aff(c, P);   // y of even parity
+            aff* c = reinterpret_cast<aff*>(cache.entries) + 2u * key_hash(x, cache.mask >> 1) + 1u;
+            store_aff(c, P);   // new keys go to way 1; y of even parity
         }
         if (parity) fe_neg(P.y, P.y);
         fe_normalize(P.y);

@@ -99,8 +99,12 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device_noalloc(ufsecp_gpu_ctx* ctx,
                                                             const uint8_t* d_points33, size_t n, uint8_t* d_out33,
                                                             int32_t* d_status, int window_bits, void* stream);
 
-/* Device and pinned buffers the library has allocated (grown) since it was
- * loaded: a test hook for allocation-free entry points. */
+/* Device and pinned allocations the library has made since it was loaded:
+ * its own buffers and those of ufsecp_gpu_ext_alloc / _host_alloc.  A test
+ * hook for allocation-free entry points and for failure injection: with
+ * UFSECP_GPU_FAIL_ALLOC=K in the environment (read at each allocation), the
+ * allocation that brings this count to K fails as if the device were out of
+ * memory, and the call that made it returns UFSECP_ERR_GPU_MEMORY. */
 UFSECP_API uint64_t ufsecp_gpu_ext_buffer_alloc_count(void);
 
 /* ufsecp_gpu_ext_msm_device over bases already decompressed: d_points64 holds

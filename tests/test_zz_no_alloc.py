@@ -40,6 +40,8 @@ def test_msm_noalloc_matches_msm_device_two_calls(gpu):
         sc, pts, ks = wl.generate_msm(n, seed=seed, threads=8)
         jobs.append((n, c, gpu.to_device(sc), gpu.to_device(pts), compress(mul(wl.msm_closed_form_scalar(sc, ks)))))
     d_o = gpu.empty(64)
+    # the test's own device buffers are allocated before the count is taken
+    bad = gpu.to_device(np.frombuffer(b"\x05" + bytes(32), np.uint8).reshape(1, 33).repeat(3001, 0))
     for n, c, d_s, d_p, want in jobs:
         assert _regular(gpu, d_s, d_p, n, c) == (0, want)      # warms the regular path's own buffers
     before = gpu.lib.ufsecp_gpu_ext_buffer_alloc_count()
@@ -49,7 +51,6 @@ def test_msm_noalloc_matches_msm_device_two_calls(gpu):
     assert gpu.lib.ufsecp_gpu_ext_buffer_alloc_count() == before
     # an invalid term in a call after valid ones: the error word starts over
     n, c, d_s, d_p, want = jobs[1]
-    bad = gpu.to_device(np.frombuffer(b"\x05" + bytes(32), np.uint8).reshape(1, 33).repeat(n, 0))
     d_o.upload(np.full(64, 0xCC, np.uint8))
     assert _noalloc(gpu, d_s, bad, n, d_o)[:2] == (UFSECP_OK, 3)
     d_o.upload(np.full(64, 0xCC, np.uint8))

@@ -13,13 +13,36 @@ static std::atomic<uint64_t> g_buffer_allocs{0};
 
 uint64_t buffer_alloc_count() { return g_buffer_allocs.load(std::memory_order_relaxed); }
 
+static bool injected_failure(uint64_t ordinal) {
+    const char* s = std::getenv("UFSECP_GPU_FAIL_ALLOC");
+    if (!s || !*s) return false;
+    char* end = nullptr;
+    const unsigned long long k = std::strtoull(s, &end, 10);
+    return end && *end == 0 && k == ordinal;
+}
+
+hipError_t dev_malloc_raw(void** p, size_t bytes) {
+    *p = nullptr;
+    if (injected_failure(g_buffer_allocs.fetch_add(1, std::memory_order_relaxed) + 1)) return hipErrorOutOfMemory;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) *p = nullptr;
+    return e;
+}
+
+hipError_t pinned_malloc_raw(void** p, size_t bytes, unsigned flags) {
+    *p = nullptr;
+    if (injected_failure(g_buffer_allocs.fetch_add(1, std::memory_order_relaxed) + 1)) return hipErrorOutOfMemory;
+    const hipError_t e = hipHostMalloc(p, bytes, flags);
+    if (e != hipSuccess) *p = nullptr;
+    return e;
+}
+
 hipError_t DeviceBuffer::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     release();
     size_t want = 1;
     while (want < bytes) want <<= 1;   // grow-only, power of two
-    g_buffer_allocs.fetch_add(1, std::memory_order_relaxed);
-    hipError_t e = hipMalloc(&ptr, want);
+    hipError_t e = dev_malloc(&ptr, want);
     if (e != hipSuccess) {
         ptr = nullptr;
         cap = 0;
@@ -40,8 +63,7 @@ hipError_t PinnedBuffer::ensure(size_t bytes) {
     release();
     size_t want = 1;
     while (want < bytes) want <<= 1;
-    g_buffer_allocs.fetch_add(1, std::memory_order_relaxed);
-    hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+    hipError_t e = pinned_malloc(&ptr, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         ptr = nullptr;
         cap = 0;
@@ -106,7 +128,6 @@ RocmBackend::~RocmBackend() {
     h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
-    h_stage_.release();
     if (d_gcomb_) (void)hipFree(d_gcomb_);
     if (d_mid_) (void)hipFree(d_mid_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -135,13 +156,13 @@ GpuError RocmBackend::init(uint32_t device_index) {
     tuning_ = load_tuning();
     const int w = env_int("UFSECP_GCOMB_W", tuning_.gcomb_w, 4, 20);
     const size_t entries = gcomb_entries(w);
-    e = hipMalloc(&d_gcomb_, entries * 64);
+    e = dev_malloc(&d_gcomb_, entries * 64);
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);
     e = launch_build_gcomb(d_gcomb_, w, gcomb_nwin(w), stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "build G table", e);
     gdesc_ = GcombDesc{d_gcomb_, w, gcomb_nwin(w)};
 
-    e = hipMalloc((void**)&d_mid_, 8 * sizeof(uint32_t));
+    e = dev_malloc(&d_mid_, 8 * sizeof(uint32_t));
     if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(midstate)", e);
     e = launch_bip340_midstate(d_mid_, stream_);
     if (e != hipSuccess) return fail(GpuError::Launch, "BIP-340 midstate", e);

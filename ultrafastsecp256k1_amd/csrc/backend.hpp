@@ -37,9 +37,23 @@ enum class GpuError : int {
     Queue = 106,
 };
 
-// Allocations made by DeviceBuffer / PinnedBuffer growth since the library
-// was loaded (device and pinned host memory).
+// Every device and pinned host allocation of the library goes through
+// dev_malloc / pinned_malloc.  buffer_alloc_count() is the number made since
+// the library was loaded.  Failure injection for tests of the error paths:
+// with UFSECP_GPU_FAIL_ALLOC=K in the environment (read at each allocation),
+// the allocation that would bring the count to K returns hipErrorOutOfMemory
+// without calling the runtime (it still counts).
 uint64_t buffer_alloc_count();
+hipError_t dev_malloc_raw(void** p, size_t bytes);
+hipError_t pinned_malloc_raw(void** p, size_t bytes, unsigned flags);
+template <class T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    return dev_malloc_raw(reinterpret_cast<void**>(p), bytes);
+}
+template <class T>
+hipError_t pinned_malloc(T** p, size_t bytes, unsigned flags) {
+    return pinned_malloc_raw(reinterpret_cast<void**>(p), bytes, flags);
+}
 
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -164,7 +178,6 @@ private:
     GcombDesc gdesc_{nullptr, 0, 0};
     DeviceBuffer d_in_[3];
     DeviceBuffer d_out_;
-    PinnedBuffer h_stage_;
     // multi-batch state: descriptor/tile tables, work counter, captured graph
     DeviceBuffer d_descs_, d_tiles_, d_words_;
     PinnedBuffer h_tables_;

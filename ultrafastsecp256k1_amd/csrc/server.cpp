@@ -93,14 +93,14 @@ GpuError VerifyServer::start(const ServerConfig& cfg) {
     in_ring_.cap = in_cap;
     pt_ring_.cap = pt_cap;
     out_ring_.cap = out_cap;
-    if ((e = hipHostMalloc((void**)&h_in_, in_cap, hipHostMallocDefault)) != hipSuccess) return GpuError::Memory;
-    if ((e = hipHostMalloc((void**)&h_out_, out_cap, hipHostMallocMapped)) != hipSuccess) return GpuError::Memory;
+    if ((e = pinned_malloc(&h_in_, in_cap, hipHostMallocDefault)) != hipSuccess) return GpuError::Memory;
+    if ((e = pinned_malloc(&h_out_, out_cap, hipHostMallocMapped)) != hipSuccess) return GpuError::Memory;
     if ((e = hipHostGetDevicePointer((void**)&d_out_view_, h_out_, 0)) != hipSuccess) return GpuError::Memory;
-    if ((e = hipMalloc((void**)&d_in_, in_cap)) != hipSuccess) return GpuError::Memory;
-    if ((e = hipMalloc((void**)&d_pts_, pt_cap)) != hipSuccess) return GpuError::Memory;
+    if ((e = dev_malloc(&d_in_, in_cap)) != hipSuccess) return GpuError::Memory;
+    if ((e = dev_malloc(&d_pts_, pt_cap)) != hipSuccess) return GpuError::Memory;
     if (cfg_.cache_log2) {
         const size_t entries = (size_t)1 << cfg_.cache_log2;
-        if ((e = hipMalloc(&d_cache_, entries * 64)) != hipSuccess) return GpuError::Memory;
+        if ((e = dev_malloc(&d_cache_, entries * 64)) != hipSuccess) return GpuError::Memory;
         if ((e = hipMemset(d_cache_, 0, entries * 64)) != hipSuccess) return GpuError::Memory;
         cache_mask_ = (uint32_t)(entries - 1);
     }
@@ -113,15 +113,15 @@ GpuError VerifyServer::start(const ServerConfig& cfg) {
     for (Group& g : groups_) {
         if ((e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking)) != hipSuccess) return GpuError::Queue;
         if ((e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming)) != hipSuccess) return GpuError::Backend;
-        if ((e = hipMalloc(&g.d_slots, (size_t)verify_grid_ * kVerifyBlock * kSlotBytes)) != hipSuccess)
+        if ((e = dev_malloc(&g.d_slots, (size_t)verify_grid_ * kVerifyBlock * kSlotBytes)) != hipSuccess)
             return GpuError::Memory;
-        if ((e = hipMalloc((void**)&g.d_words, 64 * sizeof(uint32_t))) != hipSuccess) return GpuError::Memory;
-        if ((e = hipMalloc((void**)&g.d_miss, (size_t)cfg_.max_group_items * 8 + 8 * max_batches_)) != hipSuccess)
+        if ((e = dev_malloc(&g.d_words, 64 * sizeof(uint32_t))) != hipSuccess) return GpuError::Memory;
+        if ((e = dev_malloc(&g.d_miss, (size_t)cfg_.max_group_items * 8 + 8 * max_batches_)) != hipSuccess)
             return GpuError::Memory;
-        if ((e = hipMalloc((void**)&g.d_tables, table_bytes_)) != hipSuccess) return GpuError::Memory;
-        if ((e = hipHostMalloc((void**)&g.h_tables, table_bytes_, hipHostMallocDefault)) != hipSuccess)
+        if ((e = dev_malloc(&g.d_tables, table_bytes_)) != hipSuccess) return GpuError::Memory;
+        if ((e = pinned_malloc(&g.h_tables, table_bytes_, hipHostMallocDefault)) != hipSuccess)
             return GpuError::Memory;
-        if ((e = hipHostMalloc((void**)&g.h_miss_count, 64, hipHostMallocDefault)) != hipSuccess)
+        if ((e = pinned_malloc(&g.h_miss_count, 64, hipHostMallocDefault)) != hipSuccess)
             return GpuError::Memory;
     }
     sched_thr_ = std::thread([this] { scheduler_loop(); });

@@ -532,7 +532,7 @@ ufsecp_error_t ufsecp_gpu_ext_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void** d_
     if (!ctx || !d_ptr_out) return UFSECP_ERR_NULL_ARG;
     *d_ptr_out = nullptr;
     if (bytes == 0) return UFSECP_ERR_BAD_INPUT;
-    if (hipMalloc(d_ptr_out, bytes) != hipSuccess) {
+    if (ufs::dev_malloc(d_ptr_out, bytes) != hipSuccess) {
         *d_ptr_out = nullptr;
         return UFSECP_ERR_GPU_MEMORY;
     }
@@ -644,7 +644,7 @@ ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void
     if (!ctx || !h_ptr_out) return UFSECP_ERR_NULL_ARG;
     *h_ptr_out = nullptr;
     if (bytes == 0) return UFSECP_ERR_BAD_INPUT;
-    if (hipHostMalloc(h_ptr_out, bytes, hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
+    if (ufs::pinned_malloc(h_ptr_out, bytes, hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
         *h_ptr_out = nullptr;
         return UFSECP_ERR_GPU_MEMORY;
     }

@@ -13,6 +13,10 @@ namespace ufs {
 constexpr int kMsmBlock = 256;
 constexpr int kReduceBlock = 256;
 constexpr uint32_t kSegChunk = 32;   // entries per lane in the bucket reduction
+constexpr uint32_t kLevelChunk = 16;  // slots per lane in the levels after the pieces
+// a level writes two slots per chunk: chunks of four or more slots shrink
+// every slot count above one chunk (with three, four slots stay four)
+static_assert(kLevelChunk >= 4, "level chunks of at least four slots");
 
 UFS_DEV void msm_load_scalar(u32 k[8], const u8* p) {
     const uint4* q = reinterpret_cast<const uint4*>(p);
@@ -290,42 +294,71 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
     opart[2 * j + 1] = tv;
 }
 
-// Higher levels: the same chunked run reduction over the compacted, still
-// bucket-ordered partials of the previous level (Jacobian additions).
+// Higher levels: the same chunked run reduction over the previous level's
+// slots, read in place (no compaction).  Slot pairs (head, tail) of
+// consecutive chunks keep bucket order; a slot is empty (kEmpty) when its
+// chunk had no partial there.  Within the partials of one bucket two
+// consecutive ones are at most one empty slot apart: a chunk that lies inside
+// a bucket's run emits its head only, a chunk where the run starts emits a
+// tail, one where it ends a head.  A chunk therefore finds the neighbour of
+// its first and last partial at most two slots away, and a level costs one
+// launch whatever is left; with nothing left every lane only reads its keys.
 __global__ void __launch_bounds__(kMsmBlock)
-k_seg(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys,
-      const jac* __restrict__ ipart, u32 K, u32 max_chunks, jac* __restrict__ bsum, u32* __restrict__ okeys,
-      jac* __restrict__ opart) {
+k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32 K, u32 max_chunks,
+        jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= max_chunks) return;
-    const u32 E = *count_p;
     const u32 beg = j * K;
+    const u32 end = min(beg + K, n);
+    u32 f = end, l = end;   // first and last non-empty slot of the chunk
+    for (u32 e = beg; e < end; ++e)
+        if (ikeys[e] != kEmpty) {
+            if (f == end) f = e;
+            l = e;
+        }
+    u32 hk = kEmpty, tk = kEmpty;
     jac hv, tv;
     jac_set_inf(hv);
     jac_set_inf(tv);
-    u32 hk = kEmpty, tk = kEmpty;
-    if (beg < E) {
-        const u32 end = min(beg + K, E);
-        const bool before = beg > 0 && ikeys[idx[beg - 1]] == ikeys[idx[beg]];
-        const bool after = end < E && ikeys[idx[end]] == ikeys[idx[end - 1]];
-        u32 cur = ikeys[idx[beg]];
+    if (f < end) {
+        const u32 kf = ikeys[f], kl = ikeys[l];
+        // the previous partial of kf's bucket, if any, is at f - 1 or f - 2
+        bool before = false;
+        if (f >= 1) {
+            u32 q = f - 1;
+            if (ikeys[q] == kEmpty && q >= 1) --q;
+            before = ikeys[q] == kf;
+        }
+        bool after = false;
+        if (l + 1 < n) {
+            u32 q = l + 1;
+            if (ikeys[q] == kEmpty && q + 1 < n) ++q;
+            after = ikeys[q] == kl;
+        }
+        u32 cur = kf;
         bool first = true;
-        jac acc;
-        jac_set_inf(acc);
-        for (u32 e = beg; e < end; ++e) {
-            const u32 s = idx[e];
-            const u32 k = ikeys[s];
+        jac26 acc;
+        jac26_set_inf(acc);
+        for (u32 e = f; e <= l; ++e) {
+            const u32 k = ikeys[e];
+            if (k == kEmpty) continue;
             if (k != cur) {
-                if (first && before) { hk = cur; hv = acc; } else { bsum[cur] = acc; }
+                jac a8;
+                jac26_to_jac(a8, acc);
+                if (first && before) { hk = cur; hv = a8; } else { bsum[cur] = a8; }
                 first = false;
                 cur = k;
-                jac_set_inf(acc);
+                jac26_set_inf(acc);
             }
-            jac_add(acc, acc, ipart[s]);
+            jac26 b;
+            jac26_from_jac(b, ipart[e]);
+            jac26_add(acc, acc, b);
         }
-        if (first && before) { hk = cur; hv = acc; }
-        else if (after) { tk = cur; tv = acc; }
-        else { bsum[cur] = acc; }
+        jac a8;
+        jac26_to_jac(a8, acc);
+        if (first && before) { hk = cur; hv = a8; }
+        else if (after) { tk = cur; tv = a8; }
+        else { bsum[cur] = a8; }
     }
     okeys[2 * j] = hk;
     opart[2 * j] = hv;
@@ -867,21 +900,22 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
                            slot_part[1]);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM pieces", e);
     }
-    size_t slots = slots0;
-    int lv = 1;
-    for (;;) {
-        const size_t chunks = (slots + K - 1) / K;
-        e = hipcub::DeviceSelect::If(scan_tmp, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
-                                     sel_count, (int)slots, SlotNotEmpty{(const uint32_t*)slot_keys[lv]}, st);
-        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
-        hipLaunchKernelGGL(k_seg, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count,
-                           (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], K, (uint32_t)chunks,
-                           bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
-        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
-        lv ^= 1;
-        if (chunks == 1) break;   // one chunk held every remaining partial: all buckets complete
-        slots = 2 * chunks;
+    // levels over the piece outputs in place (one launch each; with every
+    // bucket complete after the pieces, a level only reads its keys), until
+    // a single chunk holds every remaining slot
+    {
+        size_t slots = slots0;
+        int lv = 1;
+        for (;;) {
+            const size_t chunks = (slots + kLevelChunk - 1) / kLevelChunk;
+            hipLaunchKernelGGL(k_level, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                               (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], (uint32_t)slots,
+                               kLevelChunk, (uint32_t)chunks, bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
+            if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
+            lv ^= 1;
+            if (chunks == 1) break;
+            slots = 2 * chunks;
+        }
     }
     if ((e = launch_msm_segments(bsum, ptr[kSeg], p, st)) != hipSuccess)
         return be_->fail(GpuError::Launch, "MSM window sums", e);

@@ -107,6 +107,10 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_msm_device_noalloc(ufsecp_gpu_ctx* ctx,
  * memory, and the call that made it returns UFSECP_ERR_GPU_MEMORY. */
 UFSECP_API uint64_t ufsecp_gpu_ext_buffer_alloc_count(void);
 
+/* Device and pinned allocations of the library that are still live (made and
+ * not yet released): a leak check for tests of the error paths. */
+UFSECP_API int64_t ufsecp_gpu_ext_buffer_live_count(void);
+
 /* ufsecp_gpu_ext_msm_device over bases already decompressed: d_points64 holds
  * n * 64 bytes, x || y big-endian affine coordinates.  A point with x or y >=
  * p or off the curve is BAD_PUBKEY, like an invalid compressed point. */

@@ -19,7 +19,8 @@
 //                         the serving loop from four submitting threads, and
 //                         allocation failures injected through
 //                         UFSECP_GPU_FAIL_ALLOC at every allocation of context
-//                         creation, a verify call, an MSM and server creation.
+//                         creation, a verify call, an MSM and server creation;
+//                         at the end no allocation of the library is live.
 //
 // Prints "abi driver: <checks> checks, <failed> failed" and exits 1 if any
 // check failed.  Inputs come from libufsecp_workload (seeded synthetic
@@ -542,6 +543,8 @@ int main(int argc, char** argv) {
             serving(ctx);
             ufsecp_gpu_ctx_destroy(ctx);
             injected_failures();
+            // every context, server and buffer above has been released
+            CHECK(ufsecp_gpu_ext_buffer_live_count() == 0);
         }
     }
     std::printf("abi driver: %d checks, %d failed\n", g_checks.load(), g_failed.load());

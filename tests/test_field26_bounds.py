@@ -10,6 +10,8 @@ correctness therefore rests on limb bounds, which this test derives:
   sequence of fe26_columns with 64-bit and 32-bit overflow checks (every
   intermediate is monotone in the inputs, so the all-maximum input bounds
   every other input);
+* the same column / fold / carry sequence on actual values, against
+  a * b mod p for random and all-maximum inputs up to that magnitude;
 * limb-bound propagation through the doubling, the mixed addition and the
   general addition exactly as secp_point26.h writes them, iterated from the
   loop's entry values to a fixed point: every multiply input stays within the
@@ -94,6 +96,58 @@ def test_multiply_worst_case():
     except AssertionError:
         raised = True
     assert raised, "the multiply's limit moved: recheck MUL_MAX_MAG"
+
+
+def mul_replay(a, b):
+    """fe26_columns on actual limb values (the same columns, folds and
+    carries as the device code), returning the output limbs."""
+    h = [0] * 10
+    acc = 0
+    for k in range(10, 19):
+        for i in range(k - 9, 10):
+            acc += a[i] * b[k - i]
+        h[k - 10] = acc & M26
+        acc >>= 26
+    h[9] = acc
+    l = [0] * 10
+    acc = 0
+    for k in range(10):
+        for i in range(k + 1):
+            acc += a[i] * b[k - i]
+        acc += h[k] * R0
+        if k >= 1:
+            acc += h[k - 1] * R1
+        if k == 0:
+            acc += h[9] * R1 * R0
+        if k == 1:
+            acc += h[9] * R1 * R1
+        assert acc <= M64
+        if k < 9:
+            l[k] = acc & M26
+            acc >>= 26
+        else:
+            l[9] = acc & M22
+            acc >>= 22
+    t = acc
+    x0 = t * 977 + l[0]
+    x1 = (t << 6) + (x0 >> 26) + l[1]
+    return [x0 & M26, x1 & M26, l[2] + (x1 >> 26)] + l[3:]
+
+
+def test_multiply_replay_values():
+    """The fold algebra gives a * b mod p for inputs up to the multiply's
+    magnitude limit, all-maximum limbs included."""
+    import random
+    rnd = random.Random(2636)
+    val = lambda v: sum(x << (26 * i) for i, x in enumerate(v))
+    cases = [([m * lim for lim in LIM], [m * lim for lim in LIM]) for m in (1, 16, MUL_MAX_MAG)]
+    for _ in range(3000):
+        m1, m2 = rnd.choice((1, 2, 8, 16, MUL_MAX_MAG)), rnd.choice((1, 4, 16, MUL_MAX_MAG))
+        cases.append(([rnd.randint(0, m1 * lim) for lim in LIM], [rnd.randint(0, m2 * lim) for lim in LIM]))
+    for a, b in cases:
+        r = mul_replay(a, b)
+        assert val(r) % P == val(a) * val(b) % P
+        assert all(x <= bound for x, bound in zip(r, [M26, M26, M26 + (1 << 22)] + [M26] * 6 + [M22]))
 
 
 # --------------------------------------------------------------------------

@@ -6,8 +6,9 @@ device and pinned allocations over the process, the count that
 ufsecp_gpu_ext_buffer_alloc_count reports. Each test first finds how many
 allocations an operation makes on a fresh context, then fails each of them in
 turn. The call must return UFSECP_ERR_GPU_MEMORY with its outputs cleared
-and leave no device memory behind. The same context must then run the call
-without the knob and give exact results. The reference has no such knob;
+and leave none of the library's allocations behind (its live-allocation
+count, ufsecp_gpu_ext_buffer_live_count, returns to where it was). The same
+context must then run the call without the knob and give exact results. The reference has no such knob;
 what is checked is its ABI contract for a non-OK return: outputs cleared,
 context still usable (src/cpu/src/ufsecp_gpu_impl.cpp:129-140).
 """
@@ -41,11 +42,8 @@ def fail_nth(lib, k: int):
         os.environ.pop(KNOB, None)
 
 
-def _free_device_bytes() -> int:
-    hip = ctypes.CDLL("libamdhip64.so")
-    free, total = ctypes.c_size_t(), ctypes.c_size_t()
-    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
-    return free.value
+def _live(lib) -> int:
+    return int(lib.ufsecp_gpu_ext_buffer_live_count())
 
 
 def _buf(a):
@@ -63,7 +61,7 @@ def _sweep(call, check):
         assert rc == 0
         check(out)
     assert made >= 1
-    free0 = _free_device_bytes()
+    live0 = _live(lib)
     for k in range(1, made + 1):
         with UfsecpGpu() as g:
             with fail_nth(lib, k):
@@ -74,7 +72,7 @@ def _sweep(call, check):
             rc, out = call(g)
             assert rc == 0, f"allocation {k} of {made}: retry rc {rc}"
             check(out)
-    assert _free_device_bytes() > free0 - (16 << 20)
+    assert _live(lib) == live0, "allocations left live by the failed calls"
     return made
 
 
@@ -84,15 +82,14 @@ def test_context_creation_fails_cleanly_at_each_allocation():
     UfsecpGpu().close()
     made = _count(lib) - before
     assert made >= 4          # comb table, midstate, lane slots, work counter
-    free0 = _free_device_bytes()
+    live0 = _live(lib)
     for k in range(1, made + 1):
         ctx = ctypes.c_void_p()
         with fail_nth(lib, k):
             rc = lib.ufsecp_gpu_ctx_create(ctypes.byref(ctx), 4, 0)
         assert rc == ERR_GPU_MEMORY, k
         assert not ctx.value
-    # a leaked comb table alone would be 33.5 MB per attempt
-    assert _free_device_bytes() > free0 - (16 << 20)
+    assert _live(lib) == live0, "a failed context creation left allocations live"
     b = wl.generate("schnorr", 1000)
     with UfsecpGpu() as g:
         assert np.array_equal(g.schnorr_verify_batch(b.msgs, b.pubkeys, b.sigs), 1 - b.corrupted)
@@ -206,6 +203,7 @@ def test_multi_batch_allocation_failures():
     assert rc == 0 and made >= 1
     for b, o in zip(parts, outs):
         assert np.array_equal(o, 1 - b.corrupted)
+    live0 = _live(lib)
     for k in range(1, made + 1):
         with UfsecpGpu() as g:
             rc, _ = run(g, fail_at=k)
@@ -214,6 +212,7 @@ def test_multi_batch_allocation_failures():
             assert rc == 0
             for b, o in zip(parts, outs):
                 assert np.array_equal(o, 1 - b.corrupted), k
+    assert _live(lib) == live0, "a failed multi-batch call left allocations live"
 
 
 def test_server_creation_fails_cleanly_at_each_allocation():
@@ -228,13 +227,13 @@ def test_server_creation_fails_cleanly_at_each_allocation():
         VerifyServer(g, **cfg).close()
         made = _count(lib) - before
         assert made >= 8      # input rings, key points, cache, per-group buffers
-        free0 = _free_device_bytes()
+        live0 = _live(lib)
         for k in range(1, made + 1):
             with fail_nth(lib, k):
                 with pytest.raises(UfsecpError) as ex:
                     VerifyServer(g, **cfg)
             assert ex.value.code == ERR_GPU_MEMORY, k
-        assert _free_device_bytes() > free0 - (8 << 20)
+        assert _live(lib) == live0, "a failed server creation left allocations live"
         with VerifyServer(g, **cfg) as srv:
             outs = []
             for j in range(0, e.n, 1500):

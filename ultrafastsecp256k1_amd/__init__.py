@@ -163,6 +163,7 @@ def _bind(lib: ctypes.CDLL) -> None:
         "ufsecp_gpu_ext_msm_affine_device": (c_int, [vp, vp, vp, c_size_t, vp, vp, c_int, vp]),
         "ufsecp_gpu_ext_msm_device_noalloc": (c_int, [vp, vp, vp, c_size_t, vp, vp, c_int, vp]),
         "ufsecp_gpu_ext_buffer_alloc_count": (ctypes.c_uint64, []),
+        "ufsecp_gpu_ext_buffer_live_count": (ctypes.c_int64, []),
         "ufsecp_gpu_ext_alloc": (c_int, [vp, c_size_t, POINTER(vp)]),
         "ufsecp_gpu_ext_free": (c_int, [vp, vp]),
         "ufsecp_gpu_ext_copy_to_device": (c_int, [vp, vp, vp, c_size_t]),

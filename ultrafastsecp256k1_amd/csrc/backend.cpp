@@ -10,8 +10,10 @@
 namespace ufs {
 
 static std::atomic<uint64_t> g_buffer_allocs{0};
+static std::atomic<int64_t> g_buffer_live{0};
 
 uint64_t buffer_alloc_count() { return g_buffer_allocs.load(std::memory_order_relaxed); }
+int64_t buffer_live_count() { return g_buffer_live.load(std::memory_order_relaxed); }
 
 static bool injected_failure(uint64_t ordinal) {
     const char* s = std::getenv("UFSECP_GPU_FAIL_ALLOC");
@@ -26,6 +28,7 @@ hipError_t dev_malloc_raw(void** p, size_t bytes) {
     if (injected_failure(g_buffer_allocs.fetch_add(1, std::memory_order_relaxed) + 1)) return hipErrorOutOfMemory;
     const hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) *p = nullptr;
+    else g_buffer_live.fetch_add(1, std::memory_order_relaxed);
     return e;
 }
 
@@ -34,6 +37,21 @@ hipError_t pinned_malloc_raw(void** p, size_t bytes, unsigned flags) {
     if (injected_failure(g_buffer_allocs.fetch_add(1, std::memory_order_relaxed) + 1)) return hipErrorOutOfMemory;
     const hipError_t e = hipHostMalloc(p, bytes, flags);
     if (e != hipSuccess) *p = nullptr;
+    else g_buffer_live.fetch_add(1, std::memory_order_relaxed);
+    return e;
+}
+
+hipError_t dev_free(void* p) {
+    if (!p) return hipSuccess;
+    const hipError_t e = hipFree(p);
+    if (e == hipSuccess) g_buffer_live.fetch_sub(1, std::memory_order_relaxed);
+    return e;
+}
+
+hipError_t pinned_free(void* p) {
+    if (!p) return hipSuccess;
+    const hipError_t e = hipHostFree(p);
+    if (e == hipSuccess) g_buffer_live.fetch_sub(1, std::memory_order_relaxed);
     return e;
 }
 
@@ -53,7 +71,7 @@ hipError_t DeviceBuffer::ensure(size_t bytes) {
 }
 
 void DeviceBuffer::release() {
-    if (ptr) (void)hipFree(ptr);
+    (void)dev_free(ptr);
     ptr = nullptr;
     cap = 0;
 }
@@ -74,7 +92,7 @@ hipError_t PinnedBuffer::ensure(size_t bytes) {
 }
 
 void PinnedBuffer::release() {
-    if (ptr) (void)hipHostFree(ptr);
+    (void)pinned_free(ptr);
     ptr = nullptr;
     cap = 0;
 }
@@ -128,8 +146,8 @@ RocmBackend::~RocmBackend() {
     h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
-    if (d_gcomb_) (void)hipFree(d_gcomb_);
-    if (d_mid_) (void)hipFree(d_mid_);
+    (void)dev_free(d_gcomb_);
+    (void)dev_free(d_mid_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 

@@ -44,8 +44,13 @@ enum class GpuError : int {
 // the allocation that would bring the count to K returns hipErrorOutOfMemory
 // without calling the runtime (it still counts).
 uint64_t buffer_alloc_count();
+// Allocations made through dev_malloc / pinned_malloc and not yet released
+// through dev_free / pinned_free: a leak check for the error paths.
+int64_t buffer_live_count();
 hipError_t dev_malloc_raw(void** p, size_t bytes);
 hipError_t pinned_malloc_raw(void** p, size_t bytes, unsigned flags);
+hipError_t dev_free(void* p);      // NULL is a no-op
+hipError_t pinned_free(void* p);   // NULL is a no-op
 template <class T>
 hipError_t dev_malloc(T** p, size_t bytes) {
     return dev_malloc_raw(reinterpret_cast<void**>(p), bytes);

@@ -143,18 +143,18 @@ VerifyServer::~VerifyServer() {
         if (g.stream) (void)hipStreamSynchronize(g.stream);
         if (g.done) (void)hipEventDestroy(g.done);
         if (g.stream) (void)hipStreamDestroy(g.stream);
-        if (g.d_slots) (void)hipFree(g.d_slots);
-        if (g.d_words) (void)hipFree(g.d_words);
-        if (g.d_miss) (void)hipFree(g.d_miss);
-        if (g.d_tables) (void)hipFree(g.d_tables);
-        if (g.h_tables) (void)hipHostFree(g.h_tables);
-        if (g.h_miss_count) (void)hipHostFree(g.h_miss_count);
+        (void)dev_free(g.d_slots);
+        (void)dev_free(g.d_words);
+        (void)dev_free(g.d_miss);
+        (void)dev_free(g.d_tables);
+        (void)pinned_free(g.h_tables);
+        (void)pinned_free(g.h_miss_count);
     }
-    if (d_cache_) (void)hipFree(d_cache_);
-    if (d_pts_) (void)hipFree(d_pts_);
-    if (d_in_) (void)hipFree(d_in_);
-    if (h_out_) (void)hipHostFree(h_out_);
-    if (h_in_) (void)hipHostFree(h_in_);
+    (void)dev_free(d_cache_);
+    (void)dev_free(d_pts_);
+    (void)dev_free(d_in_);
+    (void)pinned_free(h_out_);
+    (void)pinned_free(h_in_);
     std::unordered_set<Request*> all;
     for (auto& kv : reqs_) all.insert(kv.second);
     for (Request* r : admitted_) all.insert(r);

@@ -499,6 +499,8 @@ ufsecp_error_t ufsecp_gpu_ext_msm_device_noalloc(ufsecp_gpu_ctx* ctx, const uint
 
 uint64_t ufsecp_gpu_ext_buffer_alloc_count(void) { return ufs::buffer_alloc_count(); }
 
+int64_t ufsecp_gpu_ext_buffer_live_count(void) { return ufs::buffer_live_count(); }
+
 ufsecp_error_t ufsecp_gpu_ext_msm_affine_device(ufsecp_gpu_ctx* ctx, const uint8_t* d_scalars32,
                                                 const uint8_t* d_points64, size_t n, uint8_t* d_out33,
                                                 int32_t* d_status, int window_bits, void* stream) {
@@ -543,7 +545,7 @@ ufsecp_error_t ufsecp_gpu_ext_free(ufsecp_gpu_ctx* ctx, void* d_ptr) {
     if (!ctx) return UFSECP_ERR_NULL_ARG;
     if (!d_ptr) return UFSECP_OK;
     (void)hipStreamSynchronize(ctx->backend->stream());
-    return hipFree(d_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
+    return ufs::dev_free(d_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
 }
 
 ufsecp_error_t ufsecp_gpu_ext_copy_to_device(ufsecp_gpu_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
@@ -651,7 +653,7 @@ ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void
     // kernels read this memory through the host address itself
     void* dptr = nullptr;
     if (hipHostGetDevicePointer(&dptr, *h_ptr_out, 0) != hipSuccess || dptr != *h_ptr_out) {
-        (void)hipHostFree(*h_ptr_out);
+        (void)ufs::pinned_free(*h_ptr_out);
         *h_ptr_out = nullptr;
         return UFSECP_ERR_GPU_UNSUPPORTED;
     }
@@ -661,7 +663,7 @@ ufsecp_error_t ufsecp_gpu_ext_host_alloc(ufsecp_gpu_ctx* ctx, size_t bytes, void
 ufsecp_error_t ufsecp_gpu_ext_host_free(ufsecp_gpu_ctx* ctx, void* h_ptr) {
     if (!ctx) return UFSECP_ERR_NULL_ARG;
     if (!h_ptr) return UFSECP_OK;
-    return hipHostFree(h_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
+    return ufs::pinned_free(h_ptr) == hipSuccess ? UFSECP_OK : UFSECP_ERR_GPU_MEMORY;
 }
 
 ufsecp_error_t ufsecp_gpu_ext_server_create(ufsecp_gpu_ctx* ctx, const ufsecp_gpu_server_config_t* cfg,

@@ -181,7 +181,7 @@ UFSECP_API ufsecp_error_t ufsecp_gpu_ext_host_free(ufsecp_gpu_ctx* ctx, void* h_
 typedef struct ufsecp_gpu_server ufsecp_gpu_server;
 
 typedef struct {
-    uint64_t hbm_budget_bytes;   /* device bytes for in-flight inputs + keys (0 = 256 MiB) */
+    uint64_t hbm_budget_bytes;   /* device bytes for in-flight inputs (0 = 256 MiB) */
     uint32_t inflight_groups;    /* launch groups on the device at once, 1..4 (0 = 2) */
     uint32_t key_cache_log2;     /* decompression cache of 2^k 64-byte entries; 0 = none */
     uint32_t max_group_items;    /* items coalesced into one launch (0 = 2^21) */

@@ -226,7 +226,7 @@ def test_server_creation_fails_cleanly_at_each_allocation():
         before = _count(lib)
         VerifyServer(g, **cfg).close()
         made = _count(lib) - before
-        assert made >= 8      # input rings, key points, cache, per-group buffers
+        assert made >= 8      # input and verdict rings, cache, per-group buffers
         live0 = _live(lib)
         for k in range(1, made + 1):
             with fail_nth(lib, k):

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
-GPU_SOURCES = ["kernels_verify.hip", "kernels_serve.hip", "kernels_pubdata.hip", "kernels_recover.hip",
+GPU_SOURCES = ["kernels_verify.hip", "kernels_pubdata.hip", "kernels_recover.hip",
                "kernels_rlc.hip", "msm.hip", "backend.cpp", "server.cpp", "tuning.cpp", "ufsecp_gpu_abi.cpp",
                "engine_hook.cpp", "batch_rlc.cpp", "ufsecp_gpu_unsupported.cpp"]
 

@@ -190,7 +190,7 @@ GpuError RocmBackend::init(uint32_t device_index) {
     if ((e = hipGetDeviceProperties(&p, device_)) != hipSuccess) return fail(GpuError::Device, "properties", e);
     cus_ = p.multiProcessorCount;
     unsigned max_grid = 0;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         int bpc = verify_blocks_per_cu(k);
         if (tuning_.verify_blocks_per_cu > 0 && tuning_.verify_blocks_per_cu < bpc) bpc = tuning_.verify_blocks_per_cu;
         verify_grid_[k] = (unsigned)(cus_ * bpc);

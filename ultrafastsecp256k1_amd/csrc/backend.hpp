@@ -192,7 +192,7 @@ private:
     const void* graph_descs_ = nullptr;
     const void* graph_tiles_ = nullptr;
     Tuning tuning_;
-    unsigned verify_grid_[3] = {0, 0, 0};   // blocks: ECDSA, BIP-340, mixed
+    unsigned verify_grid_[4] = {0, 0, 0, 0};   // blocks: ECDSA, BIP-340, mixed, mixed with key cache
     DeviceBuffer d_slots_;                   // kSlotBytes per lane of the largest grid
     hipEvent_t verify_done_ = nullptr;
     hipStream_t verify_stream_ = nullptr;

@@ -102,42 +102,25 @@ struct VerifyLaunch {
     const uint32_t* ntiles_dev;
     uint32_t* counter;
     void* slots;                // kSlotBytes per lane of the grid
-    // Optional public keys already decompressed (64-byte affine points per
-    // item, y = all-ones bits for an invalid key): per batch in list mode,
-    // `single_pts` in single mode; nullptr = decompress in the kernel.
-    const void* const* batch_pts;
-    const void* single_pts;
     uint32_t single_tiles;
     uint32_t collect;           // 1: valid -> zero the caller's byte, else keep
+    // Variant 3 only: the serving loop's key cache (two-way, 64-byte entries;
+    // mask = entries - 1).  Each item's key is looked up there; a miss is
+    // decompressed in the kernel and inserted.  key_hits counts the items
+    // found (zero at launch).
+    void* key_cache;
+    uint32_t key_cache_mask;
+    uint32_t* key_hits;
 };
 
 // Kernel variant: 0 = ECDSA only, 1 = BIP-340 only, 2 = kind read per batch
-// (list mode, and single batches of the opaque ECDSA kind).
+// (list mode, and single batches of the opaque ECDSA kind), 3 = as 2 with the
+// keys through the key cache (list mode).
 hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, const uint32_t* mid, unsigned grid,
                          hipStream_t st);
 int verify_blocks_per_cu(int kind);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 
-// Public-key decompression through a device-resident cache (serving path).
-// The cache is a two-way set-associative table of (x, y with even parity)
-// entries, 2^k entries of 64 bytes in 2^(k-1) sets: new keys go to way 1, a
-// key found in way 1 is copied to way 0, so keys seen once never displace
-// reused ones.  An entry is trusted only if its x matches and
-// y^2 == x^3 + 7, so concurrent writers can never produce a wrong point.
-struct KeyCacheDesc {
-    void* entries;
-    uint32_t mask;   // entries - 1
-};
-// Resolve every item of the tile list: cache hits and invalid keys are written
-// to the batch's points array; misses are appended to `miss` as (batch, item)
-// pairs, counted in *miss_count (zero at launch).
-hipError_t launch_key_resolve(const BatchDesc* descs, const TileDesc* tiles, uint32_t ntiles,
-                              const void* const* batch_pts, KeyCacheDesc cache, uint32_t* miss, uint32_t* miss_count,
-                              hipStream_t st);
-// Decompress the misses (grid-stride over *miss_count), write the points and
-// insert them into the cache.
-hipError_t launch_key_decompress(const BatchDesc* descs, const void* const* batch_pts, KeyCacheDesc cache,
-                                 const uint32_t* miss, const uint32_t* miss_count, unsigned grid, hipStream_t st);
 // Public-data key operations (kernels_pubdata.hip): one verdict byte per item.
 hipError_t launch_pubkey_validate(const uint8_t* pk33, uint32_t n, uint8_t* res, unsigned grid, hipStream_t st);
 hipError_t launch_xonly_validate(const uint8_t* x32, uint32_t n, uint8_t* res, unsigned grid, hipStream_t st);

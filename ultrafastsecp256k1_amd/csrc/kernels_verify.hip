@@ -13,6 +13,7 @@
 #include "kernels.h"
 #include "secp_ecmult.h"
 #include "secp_hash.h"
+#include "secp_keycache.h"
 
 #ifndef UFS_VERIFY_MIN_BLOCKS
 #define UFS_VERIFY_MIN_BLOCKS 3
@@ -66,14 +67,33 @@ UFS_DEV bool ecdsa_xcheck(const jac& R, const u32 r[8]) {
     return ok;
 }
 
-// Public key of an item from a pre-decompressed points array (y all-ones
-// bits marks an invalid key).
-UFS_DEV bool load_key_point(aff& P, const aff* pts, u32 i) {
-    load_aff(P, pts + i);
-    u32 m = 0xFFFFFFFFu;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) m &= P.y.v[k];
-    return m != 0xFFFFFFFFu;
+// Where a batch's public keys come from: the serving loop's key cache
+// (`cache`, variant 3), or none (decompress here).  In variants 0..2 `cache`
+// is a compile-time null and its code is not emitted.
+struct KeySrc {
+    aff* cache;
+    u32 mask;
+    u32* hits;
+};
+
+// Key of item x (big-endian words already converted), y of parity `odd`.
+// False for x >= p or x^3 + 7 not a square.  Through the cache: a hit uses
+// the cached point (counted); a miss is decompressed and inserted.
+UFS_DEV bool item_key(aff& P, const KeySrc& ks, const u32 x[8], u32 odd) {
+    if (ks.cache) {
+        if (u256_lt_p(x) && key_cache_find(P, ks.cache, ks.mask, x, odd)) {
+            atomicAdd(ks.hits, 1u);
+            return true;
+        }
+        if (!lift_x(P, x, 0u)) return false;
+        key_cache_insert(ks.cache, ks.mask, P);
+        if (odd) {
+            fe_neg(P.y, P.y);
+            fe_normalize(P.y);
+        }
+        return true;
+    }
+    return lift_x(P, x, odd);
 }
 
 // 256-bit little-endian integer from 32 bytes of any alignment.
@@ -87,13 +107,13 @@ UFS_DEV void load_le32_bytes(u32 r[8], const u8* p) {
 
 // One ECDSA verification of item i of batch d; returns the verdict.
 // `slot` is the lane's private table area (kSlotEntries affine entries);
-// `pts`, when not null, holds the batch's decompressed public keys.
+// `ks` says where the public key comes from.
 // opaque: the batch is of kKindEcdsaOpaque -- r and s are little-endian
 // 256-bit integers, strict in [1,n) (src/cpu/src/impl/ufsecp_ecdsa.cpp:105-117,
 // 128-140), and high-S is accepted: the reference normalises s to n - s,
 // which leaves x(R) unchanged.  Items are d.stride bytes apart when the
 // stride is set (rows), else packed.
-UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const KeySrc& ks,
                                  bool opaque, const sc& w_in);
 
 // s of item i as the inversion sees it: the signature's s when it is in range
@@ -114,17 +134,17 @@ UFS_DEV void ecdsa_item_s(const BatchDesc& d, u32 i, bool opaque, sc& s) {
     if (!ok) sc_set_u32(s, 1);
 }
 
-UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+UFS_DEV bool ecdsa_verify_item(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const KeySrc& ks,
                                bool opaque) {
     sc s, w;
     ecdsa_item_s(d, i, opaque, s);
     sc_inv(w, s);
-    return ecdsa_verify_item_w(d, i, G, slot, pts, opaque, w);
+    return ecdsa_verify_item_w(d, i, G, slot, ks, opaque, w);
 }
 
 // ECDSA verification of item i of batch d given w = 1/s mod n (computed by
 // the caller, possibly shared with another item).
-UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const aff* pts,
+UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff* slot, const KeySrc& ks,
                                  bool opaque, const sc& w_in) {
     u32 z_raw[8], r_raw[8], s_raw[8], x_raw[8];
     const u8* pk;
@@ -151,11 +171,7 @@ UFS_DEV bool ecdsa_verify_item_w(const BatchDesc& d, u32 i, const gcomb& G, aff*
     if (!opaque) ok &= !sc_is_high(s);
     ok &= (prefix == 2u) | (prefix == 3u);
     aff P;
-    if (pts) {
-        ok &= load_key_point(P, pts, i);
-    } else {
-        ok &= lift_x(P, x_raw, prefix & 1u);
-    }
+    ok &= item_key(P, ks, x_raw, prefix & 1u);
     if (!ok) {
         // keep the lane on the common path with harmless values
         set_generator(P);
@@ -196,7 +212,7 @@ UFS_DEV void load_sc(sc& a, const fe* p) {
 // the items run one after the other in a rolled loop (one inline copy of the
 // scalar multiplication).
 UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, const gcomb& G, aff* slot,
-                                const aff* pts, bool& v1, bool& v2) {
+                                const KeySrc& ks, bool& v1, bool& v2) {
     fe* stash = &slot[kSlotEntries - 1].x;
     {
         sc s1, s2, ss, inv, w;
@@ -215,7 +231,7 @@ UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, co
     for (int k = 0; k < m; ++k) {
         sc w;
         load_sc(w, stash + k);
-        const bool v = ecdsa_verify_item_w(d, k ? i2 : i1, G, slot, pts, false, w);
+        const bool v = ecdsa_verify_item_w(d, k ? i2 : i1, G, slot, ks, false, w);
         if (k == 0) v1 = v; else v2 = v;
     }
 }
@@ -223,7 +239,7 @@ UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, co
 // BIP-340 verification of item i of batch d up to the y-parity test: R =
 // s G - e P in Jacobian form and whether every check but y(R) even passed.
 UFS_DEV bool schnorr_verify_head(const BatchDesc& d, u32 i, const u32 mid[8], const gcomb& G, aff* slot,
-                                 const aff* pts, jac& R) {
+                                 const KeySrc& ks, jac& R) {
     u32 m_w[8], px_w[8], r_w[8], s_w[8];
     load_be_words(m_w, d.msgs + 32ull * i);
     load_be_words(px_w, d.pubs + 32ull * i);
@@ -235,11 +251,7 @@ UFS_DEV bool schnorr_verify_head(const BatchDesc& d, u32 i, const u32 mid[8], co
     u256_from_be_words(ss, s_w);
     bool ok = u256_lt_p(rr) && !u256_is_zero(ss) && u256_lt_n(ss);
     aff P;
-    if (pts) {
-        ok &= load_key_point(P, pts, i);
-    } else {
-        ok &= lift_x(P, px, 0u);
-    }
+    ok &= item_key(P, ks, px, 0u);
     sc s;
 #pragma unroll
     for (int k = 0; k < 8; ++k) s.v[k] = ss[k];
@@ -285,7 +297,7 @@ UFS_DEV bool y_is_even(const fe& Y, const fe& zi) {
 // already).  The head runs in a rolled loop: one inline copy of the
 // scalar multiplication serves both items.
 UFS_DEV void schnorr_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, const u32 mid[8], const gcomb& G,
-                                  aff* slot, const aff* pts, bool& v1, bool& v2) {
+                                  aff* slot, const KeySrc& ks, bool& v1, bool& v2) {
     // the first item's Y and Z wait in the slot's last entry while the
     // second item runs (no long-lived registers across the multiplication)
     fe* stash = &slot[kSlotEntries - 1].x;
@@ -296,7 +308,7 @@ UFS_DEV void schnorr_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, 
 #pragma unroll 1
     for (int k = 0; k < m; ++k) {
         jac R;
-        const bool v = schnorr_verify_head(d, k ? i2 : i1, mid, G, slot, pts, R);
+        const bool v = schnorr_verify_head(d, k ? i2 : i1, mid, G, slot, ks, R);
         fe z = R.z;
         if (R.inf) fe_set_u32(z, 1);
         if (k == 0) {
@@ -329,7 +341,8 @@ UFS_DEV void schnorr_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, 
 // over uneven batches (list mode: waves pull tiles from a device counter
 // until it passes the tile count, so the grid drains before the kernel
 // returns).  A tile never straddles batches, so a wave runs one signature
-// kind per tile.  KIND: 0 = ECDSA, 1 = BIP-340, 2 = per batch.
+// kind per tile.  KIND: 0 = ECDSA, 1 = BIP-340, 2 = per batch, 3 = per
+// batch with the keys through the serving loop's key cache.
 // ---------------------------------------------------------------------------
 template <int KIND>
 __global__ void __launch_bounds__(kVerifyBlock, UFS_VERIFY_MIN_BLOCKS)
@@ -353,16 +366,19 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         if (t >= ntiles) break;
         BatchDesc d;
         u32 first;
-        const aff* pts;
+        KeySrc ks{nullptr, 0u, nullptr};
+        if (KIND == 3) {
+            ks.cache = reinterpret_cast<aff*>(a.key_cache);
+            ks.mask = a.key_cache_mask;
+            ks.hits = a.key_hits;
+        }
         if (list) {
             const TileDesc tl = a.tiles[t];
             d = a.descs[tl.batch];
             first = tl.first;
-            pts = a.batch_pts ? reinterpret_cast<const aff*>(a.batch_pts[tl.batch]) : nullptr;
         } else {
             d = a.single;
             first = t * kTileItems;
-            pts = reinterpret_cast<const aff*>(a.single_pts);
         }
         // single ECDSA or BIP-340 batch: the lane's item of tile t + W joins
         // the one of tile t (shared inversion); the tile loop then steps by 2W
@@ -373,14 +389,14 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         if (i >= d.count) continue;
         const u32 i2 = t2 * kTileItems + lane;
         const bool two = pairing && t2 < ntiles && i2 < d.count;
-        const u32 kind = KIND == 2 ? d.kind : (u32)KIND;
+        const u32 kind = KIND >= 2 ? d.kind : (u32)KIND;
         bool v, v2 = false;
         if (kind == kKindSchnorr) {
-            schnorr_verify_items(d, i, i2, two, mid, G, slot, pts, v, v2);
+            schnorr_verify_items(d, i, i2, two, mid, G, slot, ks, v, v2);
         } else if (KIND == 0) {
-            ecdsa_verify_items(d, i, i2, two, G, slot, pts, v, v2);
+            ecdsa_verify_items(d, i, i2, two, G, slot, ks, v, v2);
         } else {
-            v = ecdsa_verify_item(d, i, G, slot, pts, kind == kKindEcdsaOpaque);
+            v = ecdsa_verify_item(d, i, G, slot, ks, kind == kKindEcdsaOpaque);
         }
         if (a.collect) {
             if (v) d.out[i] = 0;
@@ -613,6 +629,7 @@ hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, co
     switch (kind) {
     case 0: hipLaunchKernelGGL(k_verify<0>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     case 1: hipLaunchKernelGGL(k_verify<1>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
+    case 3: hipLaunchKernelGGL(k_verify<3>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     default: hipLaunchKernelGGL(k_verify<2>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     }
     return hipGetLastError();
@@ -624,6 +641,7 @@ int verify_blocks_per_cu(int kind) {
     switch (kind) {
     case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<0>, kVerifyBlock, 0); break;
     case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<1>, kVerifyBlock, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<3>, kVerifyBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<2>, kVerifyBlock, 0); break;
     }
     if (e != hipSuccess || blocks <= 0) return 1;

@@ -4,7 +4,7 @@
 //
 // Workload: request sizes log-uniform in [1, max_size] (floor(exp(u ln(max+1)))),
 // kind ECDSA with probability 0.7 else BIP-340; items are taken in order,
-// cyclically, from two pre-generated serving workloads in which 20% of the
+// cyclically, from two pre-generated serving workloads in which 20% (by default) of the
 // keys come from a seeded pool of 2^16 keys (so the device decompression cache
 // has something to keep) and 1% of the items are corrupted.  For each server
 // the sustained peak is measured first (a saturating backlog); then arrivals at
@@ -81,6 +81,7 @@ struct Args {
     size_t ecdsa_items = 4u << 20, schnorr_items = 2u << 20;
     uint32_t max_size = 8192;
     uint32_t cache_log2 = 20;
+    uint32_t pool_permille = 200;   // share of items whose key comes from the 2^16-key pool
     uint32_t groups = 0;
     uint64_t budget_mb = 512;
     bool plain = true;
@@ -347,6 +348,7 @@ int main(int argc, char** argv) {
         else if (s == "--schnorr-items") a.schnorr_items = std::strtoull(val(), nullptr, 10);
         else if (s == "--max-size") a.max_size = (uint32_t)std::atoi(val());
         else if (s == "--cache-log2") a.cache_log2 = (uint32_t)std::atoi(val());
+        else if (s == "--pool-permille") a.pool_permille = (uint32_t)std::atoi(val());
         else if (s == "--groups") a.groups = (uint32_t)std::atoi(val());
         else if (s == "--budget-mb") a.budget_mb = std::strtoull(val(), nullptr, 10);
         else if (s == "--no-plain") a.plain = false;
@@ -391,8 +393,10 @@ int main(int argc, char** argv) {
         p.sigs = b + 32 * p.n;
         p.pubs = p.sigs + 64 * p.n;
         p.corr = p.pubs + p.pub_stride() * p.n;
-        rc = k == 0 ? ufsecp_wl_ecdsa_pool(0x5E4A, 0, p.n, 0x9001, 1u << 16, 200, p.msgs, p.pubs, p.sigs, p.corr, 0)
-                    : ufsecp_wl_schnorr_pool(0x5E4B, 0, p.n, 0x9001, 1u << 16, 200, p.msgs, p.pubs, p.sigs, p.corr,
+        rc = k == 0 ? ufsecp_wl_ecdsa_pool(0x5E4A, 0, p.n, 0x9001, 1u << 16, a.pool_permille, p.msgs, p.pubs, p.sigs,
+                                           p.corr, 0)
+                    : ufsecp_wl_schnorr_pool(0x5E4B, 0, p.n, 0x9001, 1u << 16, a.pool_permille, p.msgs, p.pubs, p.sigs,
+                                             p.corr,
                                              0);
         if (rc) die("workload", rc);
     }
@@ -446,11 +450,11 @@ int main(int argc, char** argv) {
     for (auto& r : runs) bad += r.mismatches;
     for (auto& r : pruns) bad += r.mismatches;
     std::printf("{\"workload\": \"Poisson arrivals, sizes log-uniform [1,%u] (mean %.0f), 70/30 ECDSA/BIP-340, "
-                "20%% of keys from a 2^16 pool, 1%% corrupted\", \"mean_request_size\": %.1f, "
+                "%.1f%% of keys from a 2^16 pool, 1%% corrupted\", \"mean_request_size\": %.1f, "
                 "\"pool_items\": [%zu, %zu], \"generate_seconds\": %.1f, "
                 "\"config\": {\"hbm_budget_mb\": %llu, \"inflight_groups\": %u, \"key_cache_log2\": %u}, ",
-                a.max_size, msize, msize, pools[0].n, pools[1].n, gen_s, (unsigned long long)a.budget_mb, a.groups,
-                a.cache_log2);
+                a.max_size, msize, a.pool_permille / 10.0, msize, pools[0].n, pools[1].n, gen_s,
+                (unsigned long long)a.budget_mb, a.groups, a.cache_log2);
     std::printf("\"batched\": {\"peak_sigs_per_s\": %.0f, \"peak_run\": ", peak.achieved);
     print_run("\"peak\"", peak, 1.0, true);
     std::printf(", \"loads\": [");

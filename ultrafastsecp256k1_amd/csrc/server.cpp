@@ -82,22 +82,20 @@ GpuError VerifyServer::start(const ServerConfig& cfg) {
     device_ = be_->device();
     hipError_t e = hipSetDevice(device_);
     if (e != hipSuccess) return GpuError::Device;
-    verify_grid_ = be_->verify_grid(2);
+    verify_variant_ = cfg_.cache_log2 ? 3 : 2;
+    verify_grid_ = be_->verify_grid(verify_variant_);
 
-    // Ring sizes: the HBM budget is split between inputs (<= 129 B per item
-    // plus alignment) and decompressed keys (64 B per item).
-    const size_t in_cap = align_up(cfg_.hbm_budget / 194 * 130, kAlign);
-    const size_t items_cap = in_cap / 130;
-    const size_t pt_cap = align_up(items_cap * 64 + (1 << 20), kAlign);
+    // Ring sizes: the HBM budget holds the inputs (<= 129 B per item plus
+    // alignment); the verdict ring one byte per item.
+    const size_t in_cap = cfg_.hbm_budget / kAlign * kAlign;
+    const size_t items_cap = in_cap / 129;
     const size_t out_cap = align_up(items_cap + (1 << 20), kAlign);
     in_ring_.cap = in_cap;
-    pt_ring_.cap = pt_cap;
     out_ring_.cap = out_cap;
     if ((e = pinned_malloc(&h_in_, in_cap, hipHostMallocDefault)) != hipSuccess) return GpuError::Memory;
     if ((e = pinned_malloc(&h_out_, out_cap, hipHostMallocMapped)) != hipSuccess) return GpuError::Memory;
     if ((e = hipHostGetDevicePointer((void**)&d_out_view_, h_out_, 0)) != hipSuccess) return GpuError::Memory;
     if ((e = dev_malloc(&d_in_, in_cap)) != hipSuccess) return GpuError::Memory;
-    if ((e = dev_malloc(&d_pts_, pt_cap)) != hipSuccess) return GpuError::Memory;
     if (cfg_.cache_log2) {
         const size_t entries = (size_t)1 << cfg_.cache_log2;
         if ((e = dev_malloc(&d_cache_, entries * 64)) != hipSuccess) return GpuError::Memory;
@@ -116,12 +114,10 @@ GpuError VerifyServer::start(const ServerConfig& cfg) {
         if ((e = dev_malloc(&g.d_slots, (size_t)verify_grid_ * kVerifyBlock * kSlotBytes)) != hipSuccess)
             return GpuError::Memory;
         if ((e = dev_malloc(&g.d_words, 64 * sizeof(uint32_t))) != hipSuccess) return GpuError::Memory;
-        if ((e = dev_malloc(&g.d_miss, (size_t)cfg_.max_group_items * 8 + 8 * max_batches_)) != hipSuccess)
-            return GpuError::Memory;
         if ((e = dev_malloc(&g.d_tables, table_bytes_)) != hipSuccess) return GpuError::Memory;
         if ((e = pinned_malloc(&g.h_tables, table_bytes_, hipHostMallocDefault)) != hipSuccess)
             return GpuError::Memory;
-        if ((e = pinned_malloc(&g.h_miss_count, 64, hipHostMallocDefault)) != hipSuccess)
+        if ((e = pinned_malloc(&g.h_key_hits, 64, hipHostMallocDefault)) != hipSuccess)
             return GpuError::Memory;
     }
     sched_thr_ = std::thread([this] { scheduler_loop(); });
@@ -145,13 +141,11 @@ VerifyServer::~VerifyServer() {
         if (g.stream) (void)hipStreamDestroy(g.stream);
         (void)dev_free(g.d_slots);
         (void)dev_free(g.d_words);
-        (void)dev_free(g.d_miss);
         (void)dev_free(g.d_tables);
         (void)pinned_free(g.h_tables);
-        (void)pinned_free(g.h_miss_count);
+        (void)pinned_free(g.h_key_hits);
     }
     (void)dev_free(d_cache_);
-    (void)dev_free(d_pts_);
     (void)dev_free(d_in_);
     (void)pinned_free(h_out_);
     (void)pinned_free(h_in_);
@@ -164,27 +158,21 @@ VerifyServer::~VerifyServer() {
 
 bool VerifyServer::try_admit_locked(Request* r) {
     const size_t in_len = r->mapped ? 0 : in_layout(r->kind, r->count).len;
-    const size_t pt_len = align_up((size_t)64 * r->count, kAlign);
     const size_t out_len = align_up(r->count, kAlign);
-    size_t c_in = 0, c_pt = 0, c_out = 0;
-    FifoRing in_try = in_ring_, pt_try = pt_ring_, out_try = out_ring_;
+    size_t c_in = 0, c_out = 0;
+    FifoRing in_try = in_ring_, out_try = out_ring_;
     size_t o_in = 0;
     if (!r->mapped) {
         o_in = in_try.alloc(in_len, c_in);
         if (o_in == kNone) return false;
     }
-    const size_t o_pt = pt_try.alloc(pt_len, c_pt);
-    if (o_pt == kNone) return false;
     const size_t o_out = out_try.alloc(out_len, c_out);
     if (o_out == kNone) return false;
     in_ring_ = in_try;
-    pt_ring_ = pt_try;
     out_ring_ = out_try;
     r->in_off = o_in;
     r->in_len = in_len;
     r->in_charge = c_in;
-    r->pt_off = o_pt;
-    r->pt_charge = c_pt;
     r->out_off = o_out;
     r->out_charge = c_out;
     if (r->mapped) r->staged = true;
@@ -265,7 +253,7 @@ GpuError VerifyServer::launch_group(Group& g, std::vector<Request*>& batch) {
         if (e != hipSuccess) return GpuError::Memory;
     }
 
-    // tables: descriptors | tiles | key-point pointers | tile count
+    // tables: descriptors | tiles | tile count
     const size_t nb = batch.size();
     const size_t off_tiles = align_up(nb * sizeof(BatchDesc), kAlign);
     BatchDesc* hd = reinterpret_cast<BatchDesc*>(g.h_tables);
@@ -283,37 +271,38 @@ GpuError VerifyServer::launch_group(Group& g, std::vector<Request*>& batch) {
         }
         for (uint32_t f = 0; f < r->count; f += kTileItems) ht[nt++] = TileDesc{(uint32_t)b, f};
     }
-    const size_t off_pts = off_tiles + align_up(nt * sizeof(TileDesc), kAlign);
-    const void** hp = reinterpret_cast<const void**>(g.h_tables + off_pts);
-    for (size_t b = 0; b < nb; ++b) hp[b] = d_pts_ + batch[b]->pt_off;
-    const size_t off_cnt = off_pts + align_up(nb * sizeof(void*), kAlign);
+    const size_t off_cnt = off_tiles + align_up(nt * sizeof(TileDesc), kAlign);
     *reinterpret_cast<uint32_t*>(g.h_tables + off_cnt) = (uint32_t)nt;
     const size_t tbytes = off_cnt + sizeof(uint32_t);
     if ((e = hipMemcpyAsync(g.d_tables, g.h_tables, tbytes, hipMemcpyHostToDevice, st)) != hipSuccess)
         return GpuError::Memory;
     const BatchDesc* dd = reinterpret_cast<const BatchDesc*>(g.d_tables);
     const TileDesc* dt = reinterpret_cast<const TileDesc*>(g.d_tables + off_tiles);
-    const void* const* dp = reinterpret_cast<const void* const*>(g.d_tables + off_pts);
     const uint32_t* dn = reinterpret_cast<const uint32_t*>(g.d_tables + off_cnt);
 
-    // counters: [16] work counter, [32] miss count
+    // counters: [16] work counter, [32] key-cache hits
     if ((e = launch_u32_fill(g.d_words, 0u, 48, st)) != hipSuccess) return GpuError::Launch;
-    KeyCacheDesc kc{d_cache_, cache_mask_};
-    if ((e = launch_key_resolve(dd, dt, (uint32_t)nt, dp, kc, g.d_miss, g.d_words + 32, st)) != hipSuccess)
-        return GpuError::Launch;
-    if ((e = launch_key_decompress(dd, dp, kc, g.d_miss, g.d_words + 32, (unsigned)be_->compute_units() * 4, st)) !=
-        hipSuccess)
-        return GpuError::Launch;
     VerifyLaunch a{};
     a.descs = dd;
     a.tiles = dt;
     a.ntiles_dev = dn;
     a.counter = g.d_words + 16;
     a.slots = g.d_slots;
-    a.batch_pts = dp;
-    if ((e = launch_verify(2, a, be_->gcomb(), be_->bip340_midstate(), verify_grid_, st)) != hipSuccess)
+    // Keys are resolved inside the verify kernel, through the cache when
+    // there is one (variant 3), else each decompressed there (variant 2).
+    // A separate lookup pass with dense decompression of the misses was
+    // slower at every measured hit rate, up to 96%: a 64-item wave with any
+    // miss waits for its square root either way, and two more launches per
+    // group cost more than the square roots skipped.
+    if (d_cache_) {
+        a.key_cache = d_cache_;
+        a.key_cache_mask = cache_mask_;
+        a.key_hits = g.d_words + 32;
+    }
+    if ((e = launch_verify(verify_variant_, a, be_->gcomb(), be_->bip340_midstate(), verify_grid_, st)) !=
+        hipSuccess)
         return GpuError::Launch;
-    if ((e = hipMemcpyAsync(g.h_miss_count, g.d_words + 32, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
+    if ((e = hipMemcpyAsync(g.h_key_hits, g.d_words + 32, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
         hipSuccess)
         return GpuError::Memory;
     if ((e = hipEventRecord(g.done, st)) != hipSuccess) return GpuError::Backend;
@@ -377,7 +366,7 @@ void VerifyServer::finish_group(Group& g) {
             --outstanding_;
         }
         st_lookups_ += g.items;
-        st_hits_ += g.items - std::min<uint64_t>(g.items, *g.h_miss_count);
+        st_hits_ += d_cache_ ? std::min<uint64_t>(g.items, *g.h_key_hits) : 0;
         g.reqs.clear();
         g.busy = false;
         inflight_.erase(std::find(inflight_.begin(), inflight_.end(), &g));
@@ -390,7 +379,6 @@ void VerifyServer::finish_group(Group& g) {
             Request* r = admitted_.front();
             admitted_.pop_front();
             in_ring_.release(r->in_charge);
-            pt_ring_.release(r->pt_charge);
             out_ring_.release(r->out_charge);
             r->released = true;
             if (r->observed) delete r;

@@ -2,18 +2,18 @@
 //
 // Requests (one kind, any size) are submitted from any thread and complete
 // asynchronously.  The server
-//   * admits a request when its bytes fit in fixed device rings (inputs and
-//     decompressed keys, sized by an HBM budget given at creation) and the
-//     matching pinned host rings; the submitting thread copies the inputs into
-//     the pinned input ring, so a launch group's inputs are one contiguous
-//     range (one or two DMA copies per group, not per request);
+//   * admits a request when its bytes fit in a fixed device input ring (sized
+//     by an HBM budget given at creation) and the matching pinned host rings;
+//     the submitting thread copies the inputs into the pinned input ring, so a
+//     launch group's inputs are one contiguous range (one or two DMA copies
+//     per group, not per request);
 //   * coalesces every admitted request into the next launch group as soon as
 //     one of its group slots (stream + lane slots + tables) is free: with two
 //     or more slots the next group is queued while the current one runs, and
 //     its waves fill compute units as the current group's waves retire;
-//   * per group: resolves public keys through the device-resident
-//     decompression cache (hits skip the square root), decompresses the misses
-//     densely, then runs the persistent verify kernel over all requests;
+//   * per group: runs the persistent verify kernel over all requests, which
+//     looks each public key up in the device-resident decompression cache
+//     (hits skip the square root) and decompresses and inserts the misses;
 //   * writes verdicts into a pinned ring the kernel stores to directly, and on
 //     completion copies them to each request's output buffer;
 //   * records per-request latency (submit -> verdicts in the caller's buffer).
@@ -39,7 +39,7 @@
 namespace ufs {
 
 struct ServerConfig {
-    size_t hbm_budget = 256ull << 20;   // device bytes for inputs + decompressed keys
+    size_t hbm_budget = 256ull << 20;   // device bytes for in-flight inputs
     unsigned groups = 2;                // launch groups in flight (1..4)
     unsigned cache_log2 = 21;           // decompression cache entries (0 = none)
     uint32_t max_group_items = 1u << 21;
@@ -85,7 +85,6 @@ private:
         const uint8_t *msgs, *pubs, *sigs;
         uint8_t* out;
         size_t in_off, in_len, in_charge;    // input rings (pinned + device mirror)
-        size_t pt_off, pt_charge;            // device ring of decompressed keys
         size_t out_off, out_charge;          // pinned verdict ring
         double t_submit;
         bool mapped = false;                  // inputs read in place, no input ring
@@ -96,11 +95,10 @@ private:
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
         void* d_slots = nullptr;             // lane slots of the verify grid
-        uint32_t* d_words = nullptr;         // [0] ntiles, [16] counter, [32] miss count
-        uint32_t* d_miss = nullptr;          // (batch, item) pairs
-        uint8_t* d_tables = nullptr;         // descriptors | tiles | key pointers
+        uint32_t* d_words = nullptr;         // [16] tile counter, [32] key-cache hits
+        uint8_t* d_tables = nullptr;         // descriptors | tiles | tile count
         uint8_t* h_tables = nullptr;         // pinned staging of the same
-        uint32_t* h_miss_count = nullptr;    // pinned copy-back of the miss count
+        uint32_t* h_key_hits = nullptr;      // pinned copy-back of the key-cache hits
         std::vector<Request*> reqs;
         uint64_t items = 0;
         bool busy = false;
@@ -118,15 +116,15 @@ private:
     RocmBackend* be_;
     ServerConfig cfg_;
     int device_ = 0;
+    int verify_variant_ = 2;            // 3 with the key cache, 2 without
     unsigned verify_grid_ = 0;
     size_t max_batches_ = 0, max_tiles_ = 0, table_bytes_ = 0;
     // rings
     uint8_t* h_in_ = nullptr;     // pinned input ring
     uint8_t* d_in_ = nullptr;     // device mirror of the input ring
-    uint8_t* d_pts_ = nullptr;    // device ring of decompressed keys (64 B per item)
     uint8_t* h_out_ = nullptr;    // pinned verdict ring (written by the kernel)
     uint8_t* d_out_view_ = nullptr;   // device address of h_out_
-    FifoRing in_ring_, pt_ring_, out_ring_;
+    FifoRing in_ring_, out_ring_;
     void* d_cache_ = nullptr;
     uint32_t cache_mask_ = 0;
     std::vector<Group> groups_;

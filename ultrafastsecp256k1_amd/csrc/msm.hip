@@ -12,8 +12,9 @@ namespace ufs {
 
 constexpr int kMsmBlock = 256;
 constexpr int kReduceBlock = 256;
-constexpr uint32_t kSegChunk = 32;   // entries per lane in the bucket reduction
-constexpr uint32_t kLevelChunk = 16;  // slots per lane in the levels after the pieces
+constexpr uint32_t kSegChunk = 32;   // entries per lane in the bucket reduction (and the layout's bound)
+constexpr uint32_t kSegChunkWide = 64;   // ... once that leaves at least two rounds of resident lanes
+constexpr uint32_t kLevelChunk = 16;  // slots per lane in the levels after the joins
 // a level writes two slots per chunk: chunks of four or more slots shrink
 // every slot count above one chunk (with three, four slots stay four)
 static_assert(kLevelChunk >= 4, "level chunks of at least four slots");
@@ -212,6 +213,9 @@ k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int
 }
 
 constexpr u32 kEmpty = 0xFFFFFFFFu;
+// Level-0 tail slot of a chunk that lies inside one bucket's run (its head
+// partial continues into the next chunk); read as an empty slot by k_level.
+constexpr u32 kCont = 0xFFFFFFFEu;
 
 __global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p, u32 n) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,9 +288,16 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
         }
         jac a8;
         jac26_to_jac(a8, acc);
-        if (first && before) { hk = cur; hv = a8; }
-        else if (after) { tk = cur; tv = a8; }
-        else { bsum[cur] = a8; }
+        if (first && before) {
+            hk = cur;
+            hv = a8;
+            if (after) tk = kCont;   // the whole chunk is inside one run
+        } else if (after) {
+            tk = cur;
+            tv = a8;
+        } else {
+            bsum[cur] = a8;
+        }
     }
     okeys[2 * j] = hk;
     opart[2 * j] = hv;
@@ -294,8 +305,44 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
     opart[2 * j + 1] = tv;
 }
 
+// Join of level-0 partials: a bucket that continues past its first chunk j
+// left a tail partial in slot 2j + 1 and a head partial in each following
+// chunk it reaches (slot 2(j + m)); the chunks strictly inside its run mark
+// their tail slot kCont.  Lane j sums such a chain when it spans at most
+// kJoinSpan chunks after j, writes the bucket and empties the chain's slots;
+// longer chains (buckets holding a large share of the terms) stay for the
+// levels.  Each head belongs to one chain, so no slot is written by two
+// lanes; a lane reads the tail slot of its chain's last chunk, which that
+// chunk's own lane may empty meanwhile -- either value reads as "not kCont".
+constexpr u32 kJoinSpan = 8;
+
+__global__ void __launch_bounds__(kMsmBlock)
+k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __restrict__ bsum) {
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= chunks) return;
+    const u32 tk = keys[2 * j + 1];
+    if (tk >= kCont) return;
+    u32 m = 1;
+    for (; m <= kJoinSpan && j + m < chunks; ++m) {
+        if (keys[2 * (j + m)] != tk) return;   // not a chain of this bucket: leave it
+        if (keys[2 * (j + m) + 1] != kCont) break;
+    }
+    if (m > kJoinSpan || j + m >= chunks) return;
+    jac26 acc, b;
+    jac26_from_jac(acc, part[2 * j + 1]);
+    for (u32 i = 1; i <= m; ++i) {
+        jac26_from_jac(b, part[2 * (j + i)]);
+        jac26_add(acc, acc, b);
+    }
+    jac r;
+    jac26_to_jac(r, acc);
+    bsum[tk] = r;
+    keys[2 * j + 1] = kEmpty;
+    for (u32 i = 1; i <= m; ++i) keys[2 * (j + i)] = kEmpty;
+}
+
 // Higher levels: the same chunked run reduction over the previous level's
-// slots, read in place (no compaction).  Slot pairs (head, tail) of
+// slots, read in place (no compaction; kCont and kEmpty are both empty).  Slot pairs (head, tail) of
 // consecutive chunks keep bucket order; a slot is empty (kEmpty) when its
 // chunk had no partial there.  Within the partials of one bucket two
 // consecutive ones are at most one empty slot apart: a chunk that lies inside
@@ -312,7 +359,7 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
     const u32 end = min(beg + K, n);
     u32 f = end, l = end;   // first and last non-empty slot of the chunk
     for (u32 e = beg; e < end; ++e)
-        if (ikeys[e] != kEmpty) {
+        if (ikeys[e] < kCont) {
             if (f == end) f = e;
             l = e;
         }
@@ -326,13 +373,13 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
         bool before = false;
         if (f >= 1) {
             u32 q = f - 1;
-            if (ikeys[q] == kEmpty && q >= 1) --q;
+            if (ikeys[q] >= kCont && q >= 1) --q;
             before = ikeys[q] == kf;
         }
         bool after = false;
         if (l + 1 < n) {
             u32 q = l + 1;
-            if (ikeys[q] == kEmpty && q + 1 < n) ++q;
+            if (ikeys[q] >= kCont && q + 1 < n) ++q;
             after = ikeys[q] == kl;
         }
         u32 cur = kf;
@@ -341,7 +388,7 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
         jac26_set_inf(acc);
         for (u32 e = f; e <= l; ++e) {
             const u32 k = ikeys[e];
-            if (k == kEmpty) continue;
+            if (k >= kCont) continue;
             if (k != cur) {
                 jac a8;
                 jac26_to_jac(a8, acc);
@@ -366,63 +413,6 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
     opart[2 * j + 1] = tv;
 }
 
-// Run-aligned pieces over the compacted level-0 partials (bucket order):
-// a run is the partials of one bucket; a piece is up to kPieceLen consecutive
-// partials of one run.  A lane per piece sums it; a piece that is a whole run
-// completes its bucket, the pieces of longer runs go on to the chunked
-// levels.  Level-0 runs are short (a bucket spans two or three chunks) except
-// for buckets holding a large share of all terms, so this one step usually
-// finishes every bucket with two or three additions per lane.
-constexpr u32 kPieceLen = 16;
-
-// rs[i] = i where partial i starts a run, else 0 (max-scanned into the run
-// start of every partial).
-__global__ void __launch_bounds__(kMsmBlock)
-k_run_heads(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys, u32 S,
-            u32* __restrict__ rs) {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S) return;
-    const u32 E = *count_p;
-    u32 v = 0;
-    if (i < E && (i == 0 || ikeys[idx[i]] != ikeys[idx[i - 1]])) v = i;
-    rs[i] = v;
-}
-
-__global__ void __launch_bounds__(kMsmBlock)
-k_piece_reduce(const u32* __restrict__ idx, const u32* __restrict__ count_p, const u32* __restrict__ ikeys,
-               const jac* __restrict__ ipart, const u32* __restrict__ pieces, const u32* __restrict__ npieces_p,
-               const u32* __restrict__ rstart, u32 S, jac* __restrict__ bsum, u32* __restrict__ okeys,
-               jac* __restrict__ opart) {
-    const u32 p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= S) return;
-    const u32 P = *npieces_p;
-    if (p >= P) {
-        okeys[p] = kEmpty;
-        return;
-    }
-    const u32 E = *count_p;
-    const u32 beg = pieces[p];
-    const u32 end = p + 1 < P ? pieces[p + 1] : E;
-    const u32 key = ikeys[idx[beg]];
-    jac26 a26;
-    jac26_from_jac(a26, ipart[idx[beg]]);
-    for (u32 j = beg + 1; j < end; ++j) {
-        jac26 b26;
-        jac26_from_jac(b26, ipart[idx[j]]);
-        jac26_add(a26, a26, b26);
-    }
-    jac acc;
-    jac26_to_jac(acc, a26);
-    const bool whole = rstart[beg] == beg && (end == E || ikeys[idx[end]] != key);
-    if (whole) {
-        bsum[key] = acc;
-        okeys[p] = kEmpty;
-    } else {
-        okeys[p] = key;
-        opart[p] = acc;
-    }
-}
-
 // r = m * a for a small non-negative integer m (double-and-add).
 UFS_DEV void jac26_mul_small(jac26& r, const jac26& a, u32 m) {
     jac26 acc;
@@ -438,17 +428,21 @@ UFS_DEV void jac26_mul_small(jac26& r, const jac26& a, u32 m) {
     r = acc;
 }
 
-// Lane (w, s): P = sum_{j<L} (s*L + j + 1) * B[w][s*L + j].  The lanes are
-// few (under one wave per SIMD) and each runs a chain of dependent point
-// additions, so the time follows the instruction count: 10 x 26-bit limbs.
+// Lane (w, s): P = sum_{j<L} (s*L + j + 1) * B[w][s*L + j] for the groups
+// w0 .. w0 + Gn - 1; group gtop holds magnitudes B + 1 .. 2B of the top
+// window.  The lanes are few (under one wave per SIMD) and each runs a chain
+// of dependent point additions, so the time follows the instruction count:
+// 10 x 26-bit limbs.  These waves raise their issue priority: when the high
+// windows' sums run beside the low windows' bucket accumulation (a second
+// stream), the short chains go first on a shared SIMD.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 G, u32 B, u32 L) {
+k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 w0, u32 Gn, u32 gtop, u32 B, u32 L) {
+    __builtin_amdgcn_s_setprio(3);
     const u32 T = B / L;
     const u32 id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= G * T) return;
-    const u32 w = id / T, s = id % T;
-    // group G - 1 holds magnitudes B + 1 .. 2B of the top window
-    const u32 off = s * L + (w == G - 1 ? B : 0u);
+    if (id >= Gn * T) return;
+    const u32 w = w0 + id / T, s = id % T;
+    const u32 off = s * L + (w == gtop ? B : 0u);
     const jac* bb = bsum + (size_t)w * B + (size_t)s * L;
     jac26 R, S;
     jac26_set_inf(R);
@@ -470,6 +464,7 @@ k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 G, u32 B
 // Block tree sum of `len` points per group into out[g * out_len + blockIdx.x].
 __global__ void __launch_bounds__(kReduceBlock)
 k_msm_reduce(const jac* __restrict__ in, jac* __restrict__ out, u32 len, u32 out_len) {
+    __builtin_amdgcn_s_setprio(3);
     __shared__ jac26 sh[kReduceBlock];
     const u32 g = blockIdx.y, q = blockIdx.x, t = threadIdx.x;
     const u32 idx = q * kReduceBlock + t;
@@ -481,7 +476,11 @@ k_msm_reduce(const jac* __restrict__ in, jac* __restrict__ out, u32 len, u32 out
     }
     sh[t] = v;
     __syncthreads();
-    for (u32 stride = kReduceBlock / 2; stride > 0; stride >>= 1) {
+    // tree depth from this block's live inputs: a block of 16 takes 4 levels
+    const u32 live = min((u32)kReduceBlock, len - q * kReduceBlock);
+    u32 top = 1;
+    while (top < live) top <<= 1;
+    for (u32 stride = top / 2; stride > 0; stride >>= 1) {
         if (t < stride) {
             jac26 a = sh[t];
             const jac26 b = sh[t + stride];
@@ -497,36 +496,58 @@ k_msm_reduce(const jac* __restrict__ in, jac* __restrict__ out, u32 len, u32 out
     }
 }
 
-__global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ wsum, u32 W, u32 c, const u32* __restrict__ err,
-                            u8* __restrict__ out33, int32_t* __restrict__ status) {
+// Horner combination of the window sums rows[0 .. nwin - 1] (row w weighs
+// 2^(c w)); with `top`, row nwin is the top window's second bucket group and
+// joins row nwin - 1.  The sum is then doubled `shift` times and `extra`, if
+// given, is added.  With `part` set the kernel writes that Jacobian point
+// (the high windows' share, computed beside the low windows' buckets) and
+// nothing else; otherwise it checks the error words and writes the
+// compressed result and the status.
+__global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ rows, u32 nwin, u32 top, u32 c, u32 shift,
+                            const jac* __restrict__ extra, const u32* __restrict__ err,
+                            u8* __restrict__ out33, int32_t* __restrict__ status, jac* __restrict__ part) {
     // one wave: every lane holds the same accumulator; the doublings spread
     // their products over three lanes (jac26_dbl_lanes), lane 0 writes
+    __builtin_amdgcn_s_setprio(3);
     if (blockIdx.x != 0) return;
     const bool writer = threadIdx.x == 0;
-    const u32 e = err[0];
-    if (e != 0xFFFFFFFFu || err[1] != 0u) {
-        if (writer) {
-            for (int i = 0; i < 33; ++i) out33[i] = 0;
-            *status = e != 0xFFFFFFFFu ? ((e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */)
-                                       : 9 /* INTERNAL: a bucket id out of range */;
+    if (!part) {
+        const u32 e = err[0];
+        if (e != 0xFFFFFFFFu || err[1] != 0u) {
+            if (writer) {
+                for (int i = 0; i < 33; ++i) out33[i] = 0;
+                *status = e != 0xFFFFFFFFu ? ((e & 1u) ? 3 /* BAD_PUBKEY */ : 5 /* BAD_INPUT */)
+                                           : 9 /* INTERNAL: a bucket id out of range */;
+            }
+            return;
         }
-        return;
     }
-    // The time is the dependent-instruction latency of the (W - 1) c
-    // doublings: they run on 10 x 26-bit limbs, whose additions and
-    // subtractions are independent per limb instead of carry chains.
+    // The time is the dependent-instruction latency of the doublings: they
+    // run on 10 x 26-bit limbs, whose additions and subtractions are
+    // independent per limb instead of carry chains.
     jac26 a26, b26;
-    jac26_from_jac(a26, wsum[W - 1]);
-    jac26_from_jac(b26, wsum[W]);
-    jac26_add(a26, a26, b26);             // the top window's two bucket groups
-    for (int w = (int)W - 2; w >= 0; --w) {
+    jac26_from_jac(a26, rows[nwin - 1]);
+    if (top) {
+        jac26_from_jac(b26, rows[nwin]);
+        jac26_add(a26, a26, b26);         // the top window's two bucket groups
+    }
+    for (int w = (int)nwin - 2; w >= 0; --w) {
         for (u32 d = 0; d < c; ++d) jac26_dbl_lanes(a26);
-        jac26_from_jac(b26, wsum[w]);
+        jac26_from_jac(b26, rows[w]);
+        jac26_add(a26, a26, b26);
+    }
+    for (u32 d = 0; d < shift; ++d) jac26_dbl_lanes(a26);
+    if (extra) {
+        jac26_from_jac(b26, extra[0]);
         jac26_add(a26, a26, b26);
     }
     jac acc;
     jac26_to_jac(acc, a26);
     if (!writer) return;
+    if (part) {
+        part[0] = acc;
+        return;
+    }
     if (acc.inf) {
         for (int i = 0; i < 33; ++i) out33[i] = 0;
         *status = 7;   // ERR_ARITH: result is the point at infinity
@@ -547,22 +568,6 @@ __global__ void k_u32_fill(u32* p, u32 v, size_t n) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-// Select predicate: slot i holds a partial (its key is not kEmpty).
-struct SlotNotEmpty {
-    const uint32_t* keys;
-    __host__ __device__ bool operator()(const uint32_t& i) const { return keys[i] != kEmpty; }
-};
-
-// Select predicate: compacted partial i starts a piece (every kPieceLen-th
-// partial of its run, counted from the run start).
-struct PieceStart {
-    const uint32_t* rstart;
-    const uint32_t* count;
-    __host__ __device__ bool operator()(const uint32_t& i) const {
-        return i < *count && (i - rstart[i]) % kPieceLen == 0;
-    }
-};
-
 static inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
@@ -584,11 +589,13 @@ hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, b
     return hipGetLastError();
 }
 
-hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st) {
+hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, uint32_t w0, uint32_t gn,
+                               hipStream_t st) {
     const uint32_t T = (uint32_t)(p.buckets / p.seg_len);
-    const uint32_t lanes = (uint32_t)p.groups * T;
+    const uint32_t lanes = gn * T;
     hipLaunchKernelGGL(k_msm_segments, dim3(blocks_for(lanes, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                       (const jac*)bsum, (jac*)seg, (uint32_t)p.groups, (uint32_t)p.buckets, (uint32_t)p.seg_len);
+                       (const jac*)bsum, (jac*)seg, w0, gn, (uint32_t)p.groups - 1u, (uint32_t)p.buckets,
+                       (uint32_t)p.seg_len);
     return hipGetLastError();
 }
 
@@ -599,10 +606,10 @@ hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_msm_final(const void* wsum, const MsmPlan& p, const uint32_t* err, uint8_t* out33,
-                            int32_t* status, hipStream_t st) {
-    hipLaunchKernelGGL(k_msm_final, dim3(1), dim3(64), 0, st, (const jac*)wsum, (uint32_t)p.windows, (uint32_t)p.c,
-                       err, out33, status);
+hipError_t launch_msm_final(const void* rows, uint32_t nwin, bool top, uint32_t c, uint32_t shift, const void* extra,
+                            const uint32_t* err, uint8_t* out33, int32_t* status, void* part, hipStream_t st) {
+    hipLaunchKernelGGL(k_msm_final, dim3(1), dim3(64), 0, st, (const jac*)rows, nwin, top ? 1u : 0u, c, shift,
+                       (const jac*)extra, err, out33, status, (jac*)part);
     return hipGetLastError();
 }
 
@@ -678,7 +685,17 @@ GpuError MsmEngine::init() {
     if (e != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", e);
     if ((e = msm_noalloc_setup(be_->device())) != hipSuccess)
         return be_->fail(GpuError::Device, "MSM scratch symbol address", e);
+    if ((e = ensure_aux()) != hipSuccess) return be_->fail(GpuError::Backend, "MSM second stream", e);
     return GpuError::Ok;
+}
+
+// The split form's second stream (default priority) and its two events.
+hipError_t MsmEngine::ensure_aux() {
+    hipError_t e = hipSuccess;
+    if (!aux_) e = hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking);
+    if (e == hipSuccess && !fork_) e = hipEventCreateWithFlags(&fork_, hipEventDisableTiming);
+    if (e == hipSuccess && !join_) e = hipEventCreateWithFlags(&join_, hipEventDisableTiming);
+    return e;
 }
 
 GpuError MsmEngine::msm_device_noalloc(const uint8_t* d_scalars32, const uint8_t* d_points33, size_t n,
@@ -701,7 +718,7 @@ GpuError MsmEngine::msm_device_noalloc(const uint8_t* d_scalars32, const uint8_t
     for (int i = 0; i < MsmLayout::kParts; ++i) ptr[i] = base + L.off[i];
     hipError_t e = order(st);
     if (e != hipSuccess) return be_->fail(GpuError::Backend, "MSM stream order", e);
-    r = run_msm(L, p, ptr, d_scalars32, d_points33, false, n, d_out33, d_status, st);
+    r = run_msm(L, p, ptr, d_scalars32, d_points33, false, n, d_out33, d_status, false, st);
     e = finish(st);
     if (r == GpuError::Ok && e != hipSuccess) return be_->fail(GpuError::Backend, "MSM completion event", e);
     if (r == GpuError::Ok) be_->clear_error();
@@ -737,10 +754,16 @@ MsmEngine::~MsmEngine() {
         (void)hipEventSynchronize(done_);
         (void)hipEventDestroy(done_);
     }
+    if (aux_) {
+        (void)hipStreamSynchronize(aux_);
+        (void)hipStreamDestroy(aux_);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
     DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &entries_,
-                           &bsum_, &seg_, &red_a_, &red_b_, &err_, &scan_tmp_, &out_, &slot_keys_[0],
-                           &slot_keys_[1], &slot_part_[0], &slot_part_[1], &sel_idx_, &sel_count_,
-                           &rlc_in_, &rlc_terms_, &rlc_part_, &run_, &sorted_, &sort_tmp_};
+                           &bsum_, &seg_, &red_a_, &red_b_, &err_, &out_, &slot_keys_[0],
+                           &slot_keys_[1], &slot_part_[0], &slot_part_[1],
+                           &rlc_in_, &rlc_terms_, &rlc_part_, &sorted_, &sort_tmp_};
     for (DeviceBuffer* b : all) b->release();
 }
 
@@ -769,26 +792,15 @@ GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLa
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac), S = L.slots0;
-    // hipcub temporary storage: the selects and the max-scan share one area
-    size_t sel_bytes = 0, if_bytes = 0, max_bytes = 0, sort_bytes = 0;
-    uint32_t* nul = nullptr;
-    hipError_t e = hipcub::DeviceSelect::If(nullptr, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), nul, nul,
-                                            (int)S, PieceStart{nul, nul}, st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceSelect::If(nullptr, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), nul, nul, (int)S,
-                                     SlotNotEmpty{nul}, st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceScan::InclusiveScan(nullptr, max_bytes, nul, nul, hipcub::Max(), (int)S, st);
-    if (e == hipSuccess)
-        e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                              (int)L.max_entries, 32, 32 + p.c, st);
+    // hipcub temporary storage of the bucket sort
+    size_t sort_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)nullptr,
+                                                     (uint64_t*)nullptr, (int)L.max_entries, 32, 32 + p.c, st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM temporary-storage size query", e);
-    L.scan_bytes = std::max(sel_bytes, std::max(if_bytes, max_bytes));
     L.sort_bytes = sort_bytes;
     const size_t sizes[MsmLayout::kParts] = {
         2 * n * sizeof(aff), 2 * n * sizeof(sc), L.max_entries * 8, L.max_entries * 8, nbk * jsz, nseg * jsz,
-        nseg * jsz, nseg * jsz, 8, S * 4, S * 4, S * jsz, S * jsz, S * 4, 16, S * 12, L.scan_bytes + 16,
-        L.sort_bytes + 16};
+        nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S * 4, S * jsz, S * jsz, L.sort_bytes + 16};
     size_t off = 0;
     for (int i = 0; i < MsmLayout::kParts; ++i) {
         L.size[i] = sizes[i];
@@ -808,131 +820,179 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
     // the engine's own grow-only buffers, one per part of the layout
     DeviceBuffer* bufs[MsmLayout::kParts] = {&pts_, &scl_, &entries_, &sorted_, &bsum_, &seg_, &red_a_, &red_b_,
                                               &err_, &slot_keys_[0], &slot_keys_[1], &slot_part_[0], &slot_part_[1],
-                                              &sel_idx_, &sel_count_, &run_, &scan_tmp_, &sort_tmp_};
+                                              &sort_tmp_};
     void* ptr[MsmLayout::kParts];
     for (int i = 0; i < MsmLayout::kParts; ++i) {
         hipError_t e = grow(*bufs[i], L.size[i]);
         if (e != hipSuccess) return be_->fail(GpuError::Memory, "hipMalloc(MSM buffers)", e);
         ptr[i] = bufs[i]->ptr;
     }
-    return run_msm(L, p, ptr, d_scalars32, d_points, affine, n, d_out33, d_status, st);
+    // from 2^msm_split_from terms on (tuning file), the high windows' sums and
+    // Horner share run on a second stream beside the low windows' buckets
+    const int from = be_->tuning().msm_split_from;
+    const bool split = from < 40 && n >= ((size_t)1 << from) && p.windows >= 2 && ensure_aux() == hipSuccess;
+    return run_msm(L, p, ptr, d_scalars32, d_points, affine, n, d_out33, d_status, split, st);
 }
 
-// The kernel sequence of one MSM over the buffers ptr[] of layout L.  Every
-// buffer a kernel reads is written earlier in the same sequence (the error
-// words and bucket sums by fill kernels), so no buffer needs to start zeroed.
-GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,
-                            const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33, int32_t* d_status,
-                            hipStream_t st) {
-    enum { kPts, kScl, kEntries, kSorted, kBsum, kSeg, kRedA, kRedB, kErr, kKeys0, kKeys1, kPart0, kPart1, kSelIdx,
-           kSelCount, kRun, kScanTmp, kSortTmp };
-    const size_t nbk = (size_t)p.groups * p.buckets;
-    const uint32_t K = kSegChunk;
-    const size_t max_entries = L.max_entries, chunks0 = L.chunks0, slots0 = L.slots0;
+namespace {
+// the parts of MsmLayout, in order
+enum MsmPart { kPts, kScl, kEntries, kSorted, kBsum, kSeg, kRedA, kRedB, kErr, kKeys0, kKeys1, kPart0, kPart1,
+               kSortTmp };
+// the high windows' share sits after the two error words in the kErr part
+constexpr size_t kPartOffset = 64;
+}  // namespace
+
+// Buckets of windows [wa, wb): their entries are the window-major range
+// [wa 2n, wb 2n) of the unsorted list; they are sorted into the same range,
+// accumulated by chunks (level 0), joins and levels into bsum.  The slot and
+// sort buffers are reused by the next call on the same stream.
+GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* const* ptr, size_t n, int wa, int wb,
+                                 hipStream_t st) {
+    const size_t n2 = 2 * n;
+    const size_t off = (size_t)wa * n2, E = (size_t)(wb - wa) * n2;
+    // k_seg0 runs two waves per SIMD: 512 lanes per CU.  Chunks of 64 halve
+    // the partials to join and the slots to scan, but only pay while the grid
+    // still fills the GPU twice over (2^20 terms at c = 16 measured 3.26 ->
+    // 3.15 ms; at 2^18 and 2^16 the wider chunks were slower).
+    const size_t cus = (size_t)(be_->compute_units() > 0 ? be_->compute_units() : 256);
+    const uint32_t K = E >= (size_t)kSegChunkWide * 2 * cus * 512 ? kSegChunkWide : kSegChunk;
+    const size_t chunks0 = (E + K - 1) / K, slots0 = 2 * chunks0;
     hipError_t e;
-    const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
-    const unsigned prep_blocks = cus * 4;
-    uint32_t* err = (uint32_t*)ptr[kErr];
-    uint32_t* sel_idx = (uint32_t*)ptr[kSelIdx];
-    uint32_t* sel_count = (uint32_t*)ptr[kSelCount];
-    uint32_t* run_heads = (uint32_t*)ptr[kRun];
-    uint32_t* run_start = run_heads + slots0;
-    uint32_t* pieces = run_start + slots0;
-    uint32_t* npieces = sel_count + 1;
-    size_t sel_bytes = L.scan_bytes, max_bytes = L.scan_bytes, if_bytes = L.scan_bytes;
-    void* scan_tmp = ptr[kScanTmp];
-    void* pts = ptr[kPts];
-    void* scl = ptr[kScl];
-    uint64_t* entries = (uint64_t*)ptr[kEntries];
-    uint64_t* sorted = (uint64_t*)ptr[kSorted];
+    const uint64_t* entries = (const uint64_t*)ptr[kEntries] + off;
+    uint64_t* sorted = (uint64_t*)ptr[kSorted] + off;
     jac* bsum = (jac*)ptr[kBsum];
     uint32_t* slot_keys[2] = {(uint32_t*)ptr[kKeys0], (uint32_t*)ptr[kKeys1]};
     jac* slot_part[2] = {(jac*)ptr[kPart0], (jac*)ptr[kPart1]};
-
-    // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
-    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, pts, scl, err, prep_blocks,
-                                st)) !=
-            hipSuccess)
-        return be_->fail(GpuError::Launch, "MSM prepare", e);
-    // bucket order: unsorted window-major entries, then a stable radix sort
-    // on the c low bits of the bucket id (zero digits are skip entries)
+    // stable radix sort on the c low bits of the bucket id
     {
-        const uint32_t n2 = (uint32_t)(2 * n);
-        const int bits = p.c;   // bucket bits that differ within one window
         size_t sort_bytes = L.sort_bytes;
-        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st, (const sc*)scl,
-                           n2, entries, p.c, p.windows, err + 1);
-        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
-        e = hipcub::DeviceRadixSort::SortKeys(ptr[kSortTmp], sort_bytes, (const uint64_t*)entries,
-                                              sorted, (int)max_entries, 32, 32 + bits, st);
+        e = hipcub::DeviceRadixSort::SortKeys(ptr[kSortTmp], sort_bytes, entries, sorted, (int)E, 32, 32 + p.c, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
     }
-    // bucket sums: empty buckets stay at infinity; level 0 over all sorted
-    // entries (skips pass through), then chunked reductions of the compacted
-    // partials until a single chunk remains.
-    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, bsum,
-                       (uint32_t)nbk);
-    hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)pts,
-                       (const uint64_t*)sorted, (uint32_t)max_entries, K, (uint32_t)chunks0,
-                       bsum, slot_keys[0], slot_part[0]);
+    hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)ptr[kPts],
+                       (const uint64_t*)sorted, (uint32_t)E, K, (uint32_t)chunks0, bsum, slot_keys[0],
+                       slot_part[0]);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level 0", e);
-    // run-aligned pieces: most buckets complete here; the pieces of long runs
-    // (in slot set 1) continue through the chunked levels below
-    {
-        const size_t S = slots0;
-        e = hipcub::DeviceSelect::If(scan_tmp, if_bytes, hipcub::CountingInputIterator<uint32_t>(0), sel_idx,
-                                     sel_count, (int)S, SlotNotEmpty{(const uint32_t*)slot_keys[0]}, st);
-        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial compaction", e);
-        hipLaunchKernelGGL(k_run_heads, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys[0],
-                           (uint32_t)S, run_heads);
-        e = hipcub::DeviceScan::InclusiveScan(scan_tmp, max_bytes, run_heads, run_start, hipcub::Max(), (int)S,
-                                              st);
-        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM run starts", e);
-        e = hipcub::DeviceSelect::If(scan_tmp, sel_bytes, hipcub::CountingInputIterator<uint32_t>(0), pieces,
-                                     npieces, (int)S, PieceStart{run_start, sel_count}, st);
-        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM piece list", e);
-        hipLaunchKernelGGL(k_piece_reduce, dim3(blocks_for(S, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)sel_idx, (const uint32_t*)sel_count, (const uint32_t*)slot_keys[0],
-                           (const jac*)slot_part[0], (const uint32_t*)pieces, (const uint32_t*)npieces,
-                           (const uint32_t*)run_start, (uint32_t)S, bsum, slot_keys[1],
-                           slot_part[1]);
-        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM pieces", e);
+    // buckets spanning a few chunks (all of them for uniform scalars) are
+    // completed by joining their chunk partials
+    hipLaunchKernelGGL(k_join, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, slot_keys[0],
+                       (const jac*)slot_part[0], (uint32_t)chunks0, bsum);
+    if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial join", e);
+    // levels over what is left, read in place (one launch each; with every
+    // bucket complete a level only reads its keys), until a single chunk
+    // holds every remaining slot
+    size_t slots = slots0;
+    int lv = 0;
+    for (;;) {
+        const size_t chunks = (slots + kLevelChunk - 1) / kLevelChunk;
+        hipLaunchKernelGGL(k_level, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], (uint32_t)slots,
+                           kLevelChunk, (uint32_t)chunks, bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
+        lv ^= 1;
+        if (chunks == 1) break;
+        slots = 2 * chunks;
     }
-    // levels over the piece outputs in place (one launch each; with every
-    // bucket complete after the pieces, a level only reads its keys), until
-    // a single chunk holds every remaining slot
-    {
-        size_t slots = slots0;
-        int lv = 1;
-        for (;;) {
-            const size_t chunks = (slots + kLevelChunk - 1) / kLevelChunk;
-            hipLaunchKernelGGL(k_level, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                               (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], (uint32_t)slots,
-                               kLevelChunk, (uint32_t)chunks, bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
-            if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
-            lv ^= 1;
-            if (chunks == 1) break;
-            slots = 2 * chunks;
-        }
-    }
-    if ((e = launch_msm_segments(bsum, ptr[kSeg], p, st)) != hipSuccess)
-        return be_->fail(GpuError::Launch, "MSM window sums", e);
-    // tree-reduce segments per window until one point per window remains
-    uint32_t len = (uint32_t)(p.buckets / p.seg_len);
-    const void* cur = ptr[kSeg];
-    void* bufs[2] = {ptr[kRedA], ptr[kRedB]};
+    return GpuError::Ok;
+}
+
+// Window sums of windows [wa, wb): segments, then block trees until one point
+// per bucket group remains.  Groups wa .. wb - 1 (and W, the top window's
+// second group, when wb = W) use rows wa.. of the segment and tree buffers,
+// so the two halves of the split form never share a row.
+GpuError MsmEngine::window_phase(const MsmPlan& p, void* const* ptr, int wa, int wb, hipStream_t st,
+                                 const void** rows) {
+    const uint32_t gn = (uint32_t)(wb - wa) + (wb == p.windows ? 1u : 0u);
+    const size_t T = (size_t)(p.buckets / p.seg_len);
+    jac* seg = (jac*)ptr[kSeg] + (size_t)wa * T;
+    hipError_t e = launch_msm_segments(ptr[kBsum], seg, p, (uint32_t)wa, gn, st);
+    if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM window sums", e);
+    uint32_t len = (uint32_t)T;
+    const void* cur = seg;
+    void* bufs[2] = {(jac*)ptr[kRedA] + (size_t)wa * T, (jac*)ptr[kRedB] + (size_t)wa * T};
     int which = 0;
     while (len > 1) {
         const uint32_t out_len = (len + kReduceBlock - 1) / kReduceBlock;
-        e = launch_msm_reduce(cur, bufs[which], (uint32_t)p.groups, len, out_len, st);
+        e = launch_msm_reduce(cur, bufs[which], gn, len, out_len, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM reduce", e);
         cur = bufs[which];
         which ^= 1;
         len = out_len;
     }
-    e = launch_msm_final(cur, p, err, d_out33, d_status, st);
+    *rows = cur;
+    return GpuError::Ok;
+}
+
+// The kernel sequence of one MSM over the buffers ptr[] of layout L.  Every
+// buffer a kernel reads is written earlier in the same sequence (the error
+// words and bucket sums by fill kernels), so no buffer needs to start zeroed.
+//
+// Split form: the window sums and the final Horner chain are latency-bound
+// (a few waves, chains of dependent point operations) and would leave the
+// GPU idle.  The high windows [W/2, W) therefore go first; their sums and
+// their Horner share, doubled c W/2 times, run on the second stream while the
+// low windows' bucket phase fills the GPU on st; the last kernel adds that
+// share to the low windows' Horner sum.  The two halves touch disjoint bsum
+// ranges and disjoint rows of the segment and tree buffers.
+GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,
+                            const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33, int32_t* d_status,
+                            bool split, hipStream_t st) {
+    const size_t nbk = (size_t)p.groups * p.buckets;
+    hipError_t e;
+    const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
+    const unsigned prep_blocks = cus * 4;
+    uint32_t* err = (uint32_t*)ptr[kErr];
+    void* part = (uint8_t*)ptr[kErr] + kPartOffset;
+    jac* bsum = (jac*)ptr[kBsum];
+
+    // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
+    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
+        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, ptr[kPts], ptr[kScl], err, prep_blocks,
+                                st)) !=
+            hipSuccess)
+        return be_->fail(GpuError::Launch, "MSM prepare", e);
+    // unsorted window-major entries of every window (zero digits are skip entries)
+    {
+        const uint32_t n2 = (uint32_t)(2 * n);
+        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st,
+                           (const sc*)ptr[kScl], n2, (uint64_t*)ptr[kEntries], p.c, p.windows, err + 1);
+        if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
+    }
+    // empty buckets stay at infinity
+    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, bsum,
+                       (uint32_t)nbk);
+    if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket fill", e);
+    const void* rows = nullptr;
+    GpuError r;
+    if (!split) {
+        if ((r = bucket_phase(L, p, ptr, n, 0, p.windows, st)) != GpuError::Ok) return r;
+        if ((r = window_phase(p, ptr, 0, p.windows, st, &rows)) != GpuError::Ok) return r;
+        e = launch_msm_final(rows, (uint32_t)p.windows, true, (uint32_t)p.c, 0, nullptr, err, d_out33, d_status,
+                             nullptr, st);
+        if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM final", e);
+        return GpuError::Ok;
+    }
+    const int ws = p.windows / 2;
+    if ((r = bucket_phase(L, p, ptr, n, ws, p.windows, st)) != GpuError::Ok) return r;
+    if ((e = hipEventRecord(fork_, st)) != hipSuccess || (e = hipStreamWaitEvent(aux_, fork_, 0)) != hipSuccess)
+        return be_->fail(GpuError::Backend, "MSM fork", e);
+    // the high windows' share, on the second stream
+    const void* hrows = nullptr;
+    GpuError rh = window_phase(p, ptr, ws, p.windows, aux_, &hrows);
+    if (rh == GpuError::Ok) {
+        e = launch_msm_final(hrows, (uint32_t)(p.windows - ws), true, (uint32_t)p.c, (uint32_t)(p.c * ws), nullptr,
+                             err, nullptr, nullptr, part, aux_);
+        if (e != hipSuccess) rh = be_->fail(GpuError::Launch, "MSM high windows' share", e);
+    }
+    if ((e = hipEventRecord(join_, aux_)) != hipSuccess) return be_->fail(GpuError::Backend, "MSM join", e);
+    // the low windows on st; st then waits for the share (also when a launch
+    // failed, so that nothing later on st runs beside the second stream)
+    r = rh;
+    if (r == GpuError::Ok) r = bucket_phase(L, p, ptr, n, 0, ws, st);
+    if (r == GpuError::Ok) r = window_phase(p, ptr, 0, ws, st, &rows);
+    if ((e = hipStreamWaitEvent(st, join_, 0)) != hipSuccess) return be_->fail(GpuError::Backend, "MSM join", e);
+    if (r != GpuError::Ok) return r;
+    e = launch_msm_final(rows, (uint32_t)ws, false, (uint32_t)p.c, 0, part, err, d_out33, d_status, nullptr, st);
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM final", e);
     return GpuError::Ok;
 }

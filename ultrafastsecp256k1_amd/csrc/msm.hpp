@@ -17,10 +17,10 @@
 //      bits, which differ between the buckets of one window, so every
 //      bucket's entries end up contiguous;
 //   3. bucket accumulation load-balanced by fixed-size chunks of the sorted
-//      entries (mixed additions); the chunk-boundary partials of a bucket are
-//      summed by one lane per run of at most 16 (usually the last step), and
-//      longer runs go through chunked reduction levels, so no lane adds more
-//      than 32 terms even when one bucket holds half of all terms;
+//      entries (mixed additions); a bucket that spans a few chunks is finished
+//      by one lane joining its chunk partials (usually the last step), longer
+//      runs go through chunked reduction levels, so no lane adds more than 32
+//      terms even when one bucket holds half of all terms;
 //   4. per-window weighted sum sum_b (b+1) B_b by segments (running sums plus
 //      one small scalar multiplication per segment) and a block tree;
 //   5. Horner combination of the windows, affine normalisation, compression.
@@ -51,19 +51,21 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 // Device launchers (msm.hip)
 hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, bool affine, uint32_t n, void* pts,
                               void* scl, uint32_t* err, unsigned max_blocks, hipStream_t st);
-hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, hipStream_t st);
+// window sums of the bucket groups w0 .. w0 + gn - 1 into seg[0 .. gn * T)
+hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, uint32_t w0, uint32_t gn,
+                               hipStream_t st);
 hipError_t launch_msm_reduce(const void* in, void* out, uint32_t groups, uint32_t len, uint32_t out_len,
                              hipStream_t st);
-hipError_t launch_msm_final(const void* wsum, const MsmPlan& p, const uint32_t* err, uint8_t* out33,
-                            int32_t* status, hipStream_t st);
+hipError_t launch_msm_final(const void* rows, uint32_t nwin, bool top, uint32_t c, uint32_t shift, const void* extra,
+                            const uint32_t* err, uint8_t* out33, int32_t* status, void* part, hipStream_t st);
 hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 
 // Device buffers of one MSM, as parts of one byte range: offsets are
 // 256-byte aligned; total is the range's size.
 struct MsmLayout {
-    static constexpr int kParts = 18;
+    static constexpr int kParts = 14;
     size_t off[kParts], size[kParts];
-    size_t total, max_entries, chunks0, slots0, scan_bytes, sort_bytes;
+    size_t total, max_entries, chunks0, slots0, sort_bytes;
 };
 
 class MsmEngine {
@@ -98,9 +100,18 @@ private:
     GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
                           uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
     GpuError msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L);
+    // split: windows [W/2, W) go first and their sums and Horner share run
+    // on the engine's second stream beside the low windows' bucket phase
     GpuError run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,
                      const uint8_t* d_points, bool affine, size_t n, uint8_t* d_out33, int32_t* d_status,
-                     hipStream_t st);
+                     bool split, hipStream_t st);
+    // Sort, accumulate and reduce the buckets of windows [wa, wb) into bsum.
+    GpuError bucket_phase(const MsmLayout& L, const MsmPlan& p, void* const* ptr, size_t n, int wa, int wb,
+                          hipStream_t st);
+    // Window sums of windows [wa, wb) (and the top window's second bucket
+    // group when wb is the last window); *rows receives one point per group.
+    GpuError window_phase(const MsmPlan& p, void* const* ptr, int wa, int wb, hipStream_t st, const void** rows);
+    hipError_t ensure_aux();
     // Every call shares the scratch buffers below.  order(st) makes st wait
     // for the previous call when that ran on another stream; grow() waits on
     // the host for the previous call before a buffer it may still read is
@@ -110,11 +121,14 @@ private:
     hipError_t finish(hipStream_t st);
     RocmBackend* be_;
     hipEvent_t done_ = nullptr;
+    // second stream of the split form, with its fork and join events
+    hipStream_t aux_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
     hipStream_t last_stream_ = nullptr;
     bool pending_ = false;
     DeviceBuffer in_s_, in_p_, pts_, scl_, entries_, bsum_, seg_, red_a_, red_b_, err_,
-        scan_tmp_, out_, slot_keys_[2], slot_part_[2], sel_idx_, sel_count_;
-    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, run_, sorted_, sort_tmp_;
+        out_, slot_keys_[2], slot_part_[2];
+    DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, sorted_, sort_tmp_;
 };
 
 }  // namespace ufs

@@ -36,6 +36,7 @@ void load(Tuning& t) {
         if (!std::strcmp(key, "gcomb_w") && a >= 4 && a <= 20) t.gcomb_w = a;
         else if (!std::strcmp(key, "verify_blocks_per_cu") && a >= 0 && a <= 8) t.verify_blocks_per_cu = a;
         else if (!std::strcmp(key, "server_groups") && a >= 1 && a <= 4) t.server_groups = a;
+        else if (!std::strcmp(key, "msm_split_from") && a >= 0 && a <= 40) t.msm_split_from = a;
         else if (!std::strcmp(key, "msm_window") && got == 3 && a >= 0 && a < 40 && b >= 2 && b <= 20)
             t.msm_window[a] = b;
     }

@@ -279,3 +279,143 @@ def test_formulas_reach_a_fixed_point():
     else:
         raise AssertionError("no fixed point")
     assert _mag(X) < 6.2 and _mag(Y) < 10.2 and _mag(Z) < 5.2
+
+
+# --------------------------------------------------------------------------
+# the wave-cooperative form (secp_wide26.h): lane t of a 21-lane group holds
+# limb / column t; replayed lane by lane
+# --------------------------------------------------------------------------
+WIDE_OUT = [(1 << 26) + (1 << 10)] * 9 + [1 << 22]          # wmul output bound
+WIDE_NORM_OUT = [1 << 26] * 9 + [(1 << 22) + 33]           # wnorm output bound
+
+
+def _shr1(v):
+    """DPP wave_shr:1 within one group (lane 0 receives 0: the previous
+    group's lane 20 is zero in every operand, and the carry into it is
+    masked)."""
+    return [0] + v[:-1]
+
+
+def wmul_replay(a, b, check=True):
+    """wmul on limb lists a, b (10 limbs): columns, two carry rounds, the
+    2^260 fold with lanes 19 and 20 folded twice, two carry rounds with limb 9
+    split at 22 bits.  With `check`, every 64-bit intermediate is bounded."""
+    lim64 = (lambda x: x <= M64) if check else (lambda x: True)
+    A = list(a) + [0] * 11
+    acc = [0] * 21
+    sh = list(A)
+    for j in range(10):
+        for t in range(21):
+            assert sh[t] <= M32 and b[j] <= M32
+            acc[t] += sh[t] * b[j]
+            assert lim64(acc[t]), "column overflow"
+        if j < 9:
+            sh = _shr1(sh)
+    for _ in range(2):
+        hi = [x >> 26 for x in acc]
+        inc = _shr1(hi)
+        acc = [(x & M26) + i for x, i in zip(acc, inc)]
+    L = acc
+    assert all(x <= M32 for x in L)
+    V = R1 * L[19] + R0 * L[20]
+    V11 = R1 * L[20]
+    m = [0] * 21
+    for t in range(10):
+        m[t] = L[t] + R0 * L[t + 10] + (R1 * L[t + 9] if t >= 1 else 0)
+        if t == 0:
+            m[t] += R0 * V
+        if t == 1:
+            m[t] += R1 * V + R0 * V11
+        if t == 2:
+            m[t] += R1 * V11
+        assert lim64(m[t])
+    for _ in range(2):
+        hi = [(x >> (22 if t == 9 else 26)) for t, x in enumerate(m)]
+        inc = _shr1(hi)
+        h9 = hi[9]
+        new = [0] * 21
+        for t in range(10):
+            keep = m[t] & ((1 << (22 if t == 9 else 26)) - 1)
+            new[t] = keep + (inc[t] if t >= 1 else 0)
+            if t == 0:
+                new[t] += h9 * 977
+            if t == 1:
+                new[t] += h9 << 6
+            assert lim64(new[t])
+        m = new
+    assert all(x == 0 for x in m[10:])
+    return m[:10]
+
+
+def wnorm_replay(a):
+    assert all(x < 2**31 for x in a)
+    x = a[9] >> 22
+    v = list(a)
+    v[9] &= M22
+    v[0] += x * 977
+    v[1] += x << 6
+    for _ in range(2):
+        hi = [0 if t == 9 else v[t] >> 26 for t in range(10)]
+        inc = [0] + hi[:9]
+        v = [(v[t] if t == 9 else v[t] & M26) + inc[t] for t in range(10)]
+    return v
+
+
+def test_wide_multiply_worst_case_and_values():
+    """The wide product: no 64-bit overflow at the largest input magnitude,
+    outputs within WIDE_OUT, and a * b mod p on random and all-maximum
+    inputs; the wide normalisation within WIDE_NORM_OUT."""
+    import random
+    rnd = random.Random(2626)
+    val = lambda v: sum(x << (26 * i) for i, x in enumerate(v))
+    cases = [([m * lim for lim in LIM], [m * lim for lim in LIM]) for m in (1, 16, MUL_MAX_MAG)]
+    for _ in range(3000):
+        m1, m2 = rnd.choice((1, 2, 8, 16, MUL_MAX_MAG)), rnd.choice((1, 4, 16, MUL_MAX_MAG))
+        cases.append(([rnd.randint(0, m1 * lim) for lim in LIM], [rnd.randint(0, m2 * lim) for lim in LIM]))
+    for a, b in cases:
+        r = wmul_replay(a, b)
+        assert val(r) % P == val(a) * val(b) % P
+        assert all(x <= bound for x, bound in zip(r, WIDE_OUT)), r
+    for _ in range(3000):
+        a = [rnd.randint(0, 2**31 - 1) for _ in range(10)]
+        r = wnorm_replay(a)
+        assert val(r) % P == val(a) % P
+        assert all(x <= bound for x, bound in zip(r, WIDE_NORM_OUT)), r
+    r = wnorm_replay([2**31 - 1] * 10)
+    assert all(x <= bound for x, bound in zip(r, WIDE_NORM_OUT))
+
+
+def test_wide_doubling_and_final_additions_reach_a_fixed_point():
+    """The MSM final: wide doublings (wdbl, same K constants as the one-lane
+    doubling, wide multiply / normalisation output bounds) alternate with the
+    one-lane general addition of an exactly converted window sum.  From an
+    exactly converted entry point the state reaches a fixed point with every
+    multiply input within LOOP_MUL_MAG and every K sufficient."""
+    global OUT, NORM_OUT
+    saved = OUT, NORM_OUT
+    exact = list(LIM)
+    one = [1] + [0] * 9
+
+    def wdbl(X, Y, Z):
+        global OUT, NORM_OUT
+        OUT, NORM_OUT = WIDE_OUT, WIDE_NORM_OUT
+        try:
+            return dbl(X, Y, Z)
+        finally:
+            OUT, NORM_OUT = saved
+
+    X, Y, Z = exact, exact, _vmax(one, exact)
+    try:
+        for _ in range(12):
+            d = wdbl(X, Y, Z)
+            g = gadd(X, Y, Z, exact, exact, exact)
+            g2 = gadd(X, Y, Z, X, Y, Z)
+            Xn, Yn, Zn = _vmax(X, d[0], g[0], g2[0]), _vmax(Y, d[1], g[1], g2[1]), _vmax(Z, d[2], g[2], g2[2])
+            if (Xn, Yn, Zn) == (X, Y, Z):
+                break
+            X, Y, Z = Xn, Yn, Zn
+        else:
+            raise AssertionError("no fixed point")
+    finally:
+        OUT, NORM_OUT = saved
+    assert _mag(X) < 6.2 and _mag(Y) < 10.2 and _mag(Z) < 5.2

@@ -348,3 +348,29 @@ def test_jac26_add_all_cases(gpu):
             assert int(out[i, 16]) == 0, i
             assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
     assert doubled >= 40 and at_inf >= 40
+
+
+OP_JAC26_DBL_WIDE = 37
+
+
+@pytest.mark.gpu
+def test_jac26_dbl_wide_against_group_law(gpu):
+    """The wave-cooperative doubling of the MSM final (field products spread
+    over 21 lanes, three at a time): item i is A with Z = i + 3 doubled
+    1 + i % 7 times (infinity when i % 11 == 10), against 2^k A by the affine
+    group law; 300 items (several waves, every repeat count), G, -G and 2G
+    among them."""
+    rnd = random.Random(37)
+    n = 300
+    As = [mul(rnd.randrange(1, N)) for _ in range(n)]
+    As[0], As[1], As[2] = mul(1), mul(N - 1), mul(2)
+    out = gpu.selftest(OP_JAC26_DBL_WIDE, _aff_pack(As), _aff_pack(As))
+    for i in range(n):
+        if i % 11 == 10:
+            assert int(out[i, 16]) == 1, i
+            continue
+        exp = As[i]
+        for _ in range(1 + i % 7):
+            exp = add(exp, exp)
+        assert int(out[i, 16]) == 0, i
+        assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i

@@ -51,6 +51,10 @@ enum SelftestOp : int {
     kOpJac26Madd = 34,
     kOpJac26Dbl = 35,
     kOpJac26Add = 36,
+    // The wave-cooperative doubling (secp_wide26.h): item i is A in Jacobian
+    // form with Z = i + 3 (infinity when i % 11 == 10), doubled 1 + i % 7
+    // times by one whole wave; output affine x || y, [16] infinity.
+    kOpJac26DblWide = 37,
 };
 
 // Signature kinds of a verify batch.

@@ -14,6 +14,7 @@
 #include "secp_ecmult.h"
 #include "secp_hash.h"
 #include "secp_keycache.h"
+#include "secp_wide26.h"
 
 #ifndef UFS_VERIFY_MIN_BLOCKS
 #define UFS_VERIFY_MIN_BLOCKS 3
@@ -461,6 +462,47 @@ __global__ void k_bip340_midstate(u32* out) {
 __global__ void __launch_bounds__(64)
 k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __restrict__ out,
            u32 count, gcomb G, aff* __restrict__ slots) {
+    if (op == kOpJac26DblWide) {
+        // every lane of the wave runs each item of this block in turn
+        for (u32 j = 0; j < blockDim.x; ++j) {
+            const u32 idx = blockIdx.x * blockDim.x + j;
+            if (idx >= count) break;
+            fe ax, ay, z, z2, z3;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                ax.v[l] = a[16 * idx + l];
+                ay.v[l] = a[16 * idx + 8 + l];
+            }
+            fe_set_u32(z, idx + 3);
+            fe_sqr(z2, z);
+            fe_mul(z3, z2, z);
+            jac Aj;
+            fe_mul(Aj.x, ax, z2);
+            fe_mul(Aj.y, ay, z3);
+            Aj.z = z;
+            Aj.inf = 0;
+            jac26 a26;
+            jac26_from_jac(a26, Aj);
+            if (idx % 11 == 10) jac26_set_inf(a26);
+            jac26_dbl_wide_n(a26, 1 + idx % 7);
+            jac R;
+            jac26_to_jac(R, a26);
+            if (threadIdx.x == 0) {
+                u32* o = out + 32 * idx;
+                for (int l = 0; l < 32; ++l) o[l] = 0;
+                o[16] = R.inf;
+                if (!R.inf) {
+                    aff A;
+                    jac_to_aff(A, R);
+                    for (int l = 0; l < 8; ++l) {
+                        o[l] = A.x.v[l];
+                        o[8 + l] = A.y.v[l];
+                    }
+                }
+            }
+        }
+        return;
+    }
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     fe x, y, r;

@@ -7,6 +7,7 @@
 
 #include "msm.hpp"
 #include "secp_ecmult.h"
+#include "secp_wide26.h"
 
 namespace ufs {
 
@@ -507,7 +508,8 @@ __global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ rows, 
                             const jac* __restrict__ extra, const u32* __restrict__ err,
                             u8* __restrict__ out33, int32_t* __restrict__ status, jac* __restrict__ part) {
     // one wave: every lane holds the same accumulator; the doublings spread
-    // their products over three lanes (jac26_dbl_lanes), lane 0 writes
+    // each field product over 21 lanes, three products at a time
+    // (jac26_dbl_wide_n), the additions run in every lane, lane 0 writes
     __builtin_amdgcn_s_setprio(3);
     if (blockIdx.x != 0) return;
     const bool writer = threadIdx.x == 0;
@@ -532,11 +534,11 @@ __global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ rows, 
         jac26_add(a26, a26, b26);         // the top window's two bucket groups
     }
     for (int w = (int)nwin - 2; w >= 0; --w) {
-        for (u32 d = 0; d < c; ++d) jac26_dbl_lanes(a26);
+        jac26_dbl_wide_n(a26, c);
         jac26_from_jac(b26, rows[w]);
         jac26_add(a26, a26, b26);
     }
-    for (u32 d = 0; d < shift; ++d) jac26_dbl_lanes(a26);
+    jac26_dbl_wide_n(a26, shift);
     if (extra) {
         jac26_from_jac(b26, extra[0]);
         jac26_add(a26, a26, b26);

@@ -184,8 +184,9 @@ def test_msm_affine_bases_invalid(gpu):
 def test_msm_one_scalar_many_bases(gpu, n, c):
     """Every term with the same scalar (and a second run with a quarter of the
     terms sharing one scalar): whole windows fall into one or two buckets, so
-    the bucket partials survive the piece step and go through several
-    reduction rounds.  Against the closed form s * sum k_i (+ the rest)."""
+    the bucket partials span far more chunks than the join takes and go
+    through several level rounds.  Against the closed form s * sum k_i (+ the
+    rest)."""
     N = wl.N_ORDER
     sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 400 + c, threads=16)
     s = int.from_bytes(bytes(sc[0]), "big")

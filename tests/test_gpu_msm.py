@@ -4,6 +4,8 @@ Small cases and error precedence come from tests/golden/edge_cases.json (the
 reference CPU ABI's outputs and codes).  Larger MSMs use bases k_i*G, whose
 result has the closed form (sum s_i k_i mod n) * G.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -52,8 +54,8 @@ def test_msm_infinity_is_arith_error(gpu):
 @pytest.mark.parametrize("c", [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_msm_every_window_width(gpu, c):
     """Explicit window widths: every c gives the closed-form point, including
-    the top window's LDS counting (256 - c(W-1) <= 8 bits) and the small
-    bucket counts of c < 4."""
+    the top window's unsigned digits (128 - c(W-1) bits plus the carry) and
+    the small bucket counts of c < 4."""
     n = 777
     sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + c)
     e = wl.msm_closed_form_scalar(sc, ks)
@@ -196,3 +198,29 @@ def test_msm_one_scalar_many_bases(gpu, n, c):
     mixed = sc.copy()
     mixed[::4] = sc[0]
     assert gpu.msm_rc(mixed, pts, n, c) == (UFSECP_OK, compress(mul(wl.msm_closed_form_scalar(mixed, ks))))
+
+
+def test_msm_two_stream_form(tmp_path, monkeypatch, golden):
+    """The split form behind the tuning key msm_split_from (off by default):
+    the high windows' sums and their Horner share run on the engine's second
+    stream beside the low windows' bucket phase, and the last kernel adds the
+    share.  Enabled here from 2^10 terms, against the closed form at several
+    window widths (odd window counts included), the golden cases, the
+    infinity result and back-to-back calls on one context."""
+    from ultrafastsecp256k1_amd import UfsecpGpu
+    tuned = Path(__file__).resolve().parents[1] / "ultrafastsecp256k1_amd" / "ufsecp_tuning.txt"
+    f = tmp_path / "split_tuning.txt"
+    f.write_text(tuned.read_text() + "\nmsm_split_from 10\n")
+    monkeypatch.setenv("UFSECP_TUNING_FILE", str(f))
+    with UfsecpGpu(0) as g:
+        for n, c in ((1 << 10, 0), (5000, 13), (3000, 7), (1 << 16, 16), (1 << 16, 15), (1 << 12, 18)):
+            sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 900 + n + c)
+            assert g.msm_rc(sc, pts, n, c) == (UFSECP_OK, compress(mul(wl.msm_closed_form_scalar(sc, ks)))), (n, c)
+        for case in golden["edge_cases"]["msm"]:
+            rc, out = g.msm_rc(bytes.fromhex(case["scalars"]), bytes.fromhex(case["points"]), case["n"])
+            assert rc == case["rc"], case["name"]
+        sc, pts, ks = wl.generate_msm(2048, seed=11)
+        e = wl.msm_closed_form_scalar(sc[:2047], ks[:2047])
+        k_last = int.from_bytes(bytes(ks[2047]), "big")
+        sc[2047] = np.frombuffer(((-e) * pow(k_last, -1, N) % N).to_bytes(32, "big"), np.uint8)
+        assert g.msm_rc(sc, pts, 2048, 11) == (UFSECP_ERR_ARITH, b"\x00" * 33)

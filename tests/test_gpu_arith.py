@@ -374,3 +374,43 @@ def test_jac26_dbl_wide_against_group_law(gpu):
             exp = add(exp, exp)
         assert int(out[i, 16]) == 0, i
         assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
+
+
+OP_JAC26_ADD_LANES = 38
+
+
+@pytest.mark.gpu
+def test_jac26_add_lanes_all_cases(gpu):
+    """The MSM final's lane-spread general addition (one product per lane, five
+    levels) against the affine group law, with the cases of
+    test_jac26_add_all_cases: B = A (doubling), B = -A (infinity), either
+    operand at infinity, generic sums; one whole wave per item."""
+    rnd = random.Random(38)
+    As, Bs = [], []
+    n = 260
+    for i in range(n):
+        A = mul(rnd.randrange(1, N))
+        kind = i % 5
+        if kind == 0:
+            B = A
+        elif kind == 1:
+            B = (A[0], (P - A[1]) % P)
+        else:
+            B = mul(rnd.randrange(1, N))
+        As.append(A)
+        Bs.append(B)
+    out = gpu.selftest(OP_JAC26_ADD_LANES, _aff_pack(As), _aff_pack(Bs))
+    doubled = at_inf = 0
+    for i in range(n):
+        a = None if i % 11 == 10 else As[i]
+        b = None if i % 13 == 12 else Bs[i]
+        exp = add(a, b)
+        if a is not None and a == b:
+            doubled += 1
+        if exp is None:
+            at_inf += 1
+            assert int(out[i, 16]) == 1, i
+        else:
+            assert int(out[i, 16]) == 0, i
+            assert (from_limbs(out[i, :8]), from_limbs(out[i, 8:16])) == exp, i
+    assert doubled >= 40 and at_inf >= 40

@@ -55,6 +55,9 @@ enum SelftestOp : int {
     // form with Z = i + 3 (infinity when i % 11 == 10), doubled 1 + i % 7
     // times by one whole wave; output affine x || y, [16] infinity.
     kOpJac26DblWide = 37,
+    // The lane-spread general addition (jac26_add_lanes): as kOpJac26Add,
+    // one whole wave per item.
+    kOpJac26AddLanes = 38,
 };
 
 // Signature kinds of a verify batch.

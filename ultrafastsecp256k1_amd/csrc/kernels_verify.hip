@@ -462,29 +462,40 @@ __global__ void k_bip340_midstate(u32* out) {
 __global__ void __launch_bounds__(64)
 k_selftest(int op, const u32* __restrict__ a, const u32* __restrict__ b, u32* __restrict__ out,
            u32 count, gcomb G, aff* __restrict__ slots) {
-    if (op == kOpJac26DblWide) {
+    if (op == kOpJac26DblWide || op == kOpJac26AddLanes) {
         // every lane of the wave runs each item of this block in turn
         for (u32 j = 0; j < blockDim.x; ++j) {
             const u32 idx = blockIdx.x * blockDim.x + j;
             if (idx >= count) break;
-            fe ax, ay, z, z2, z3;
+            // A (and B) in Jacobian form with Z = idx + 3 (idx + 5)
+            jac26 ab[2];
+            for (int o = 0; o < 2; ++o) {
+                const u32* src = o == 0 ? a : b;
+                fe ax, ay, z, z2, z3;
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                ax.v[l] = a[16 * idx + l];
-                ay.v[l] = a[16 * idx + 8 + l];
+                for (int l = 0; l < 8; ++l) {
+                    ax.v[l] = src[16 * idx + l];
+                    ay.v[l] = src[16 * idx + 8 + l];
+                }
+                fe_set_u32(z, idx + 3 + 2 * o);
+                fe_sqr(z2, z);
+                fe_mul(z3, z2, z);
+                jac Pj;
+                fe_mul(Pj.x, ax, z2);
+                fe_mul(Pj.y, ay, z3);
+                Pj.z = z;
+                Pj.inf = 0;
+                jac26_from_jac(ab[o], Pj);
             }
-            fe_set_u32(z, idx + 3);
-            fe_sqr(z2, z);
-            fe_mul(z3, z2, z);
-            jac Aj;
-            fe_mul(Aj.x, ax, z2);
-            fe_mul(Aj.y, ay, z3);
-            Aj.z = z;
-            Aj.inf = 0;
-            jac26 a26;
-            jac26_from_jac(a26, Aj);
+            jac26 a26 = ab[0];
             if (idx % 11 == 10) jac26_set_inf(a26);
-            jac26_dbl_wide_n(a26, 1 + idx % 7);
+            if (op == kOpJac26DblWide) {
+                jac26_dbl_wide_n(a26, 1 + idx % 7);
+            } else {
+                jac26 b26 = ab[1];
+                if (idx % 13 == 12) jac26_set_inf(b26);
+                jac26_add_lanes(a26, a26, b26);
+            }
             jac R;
             jac26_to_jac(R, a26);
             if (threadIdx.x == 0) {

@@ -509,7 +509,8 @@ __global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ rows, 
                             u8* __restrict__ out33, int32_t* __restrict__ status, jac* __restrict__ part) {
     // one wave: every lane holds the same accumulator; the doublings spread
     // each field product over 21 lanes, three products at a time
-    // (jac26_dbl_wide_n), the additions run in every lane, lane 0 writes
+    // (jac26_dbl_wide_n), the additions one product per lane
+    // (jac26_add_lanes), lane 0 writes
     __builtin_amdgcn_s_setprio(3);
     if (blockIdx.x != 0) return;
     const bool writer = threadIdx.x == 0;
@@ -531,17 +532,17 @@ __global__ void __launch_bounds__(64) k_msm_final(const jac* __restrict__ rows, 
     jac26_from_jac(a26, rows[nwin - 1]);
     if (top) {
         jac26_from_jac(b26, rows[nwin]);
-        jac26_add(a26, a26, b26);         // the top window's two bucket groups
+        jac26_add_lanes(a26, a26, b26);         // the top window's two bucket groups
     }
     for (int w = (int)nwin - 2; w >= 0; --w) {
         jac26_dbl_wide_n(a26, c);
         jac26_from_jac(b26, rows[w]);
-        jac26_add(a26, a26, b26);
+        jac26_add_lanes(a26, a26, b26);
     }
     jac26_dbl_wide_n(a26, shift);
     if (extra) {
         jac26_from_jac(b26, extra[0]);
-        jac26_add(a26, a26, b26);
+        jac26_add_lanes(a26, a26, b26);
     }
     jac acc;
     jac26_to_jac(acc, a26);

@@ -268,4 +268,86 @@ UFS_DEV void jac26_dbl_lanes(jac26& a) {
     a.z = Z3;
 }
 
+// out = a + b, for operands held identically by every lane of the wave (the
+// MSM final's Horner additions), with the sixteen products of jac26_add
+// spread over lanes 0..3 in five dependent levels:
+//   1: Z1^2 (lane 0), Z2^2 (1), Z1 Z2 (2)
+//   2: U1 = X1 Z2^2 (0), U2 = X2 Z1^2 (1), Z2^3 (2), Z1^3 (3)
+//   3: S1 = Y1 Z2^3 (0), S2 = Y2 Z1^3 (1), H^2 (2), Z1 Z2 H (3)
+//   4: J = H I (0), V = U1 I (1), r^2 (2)          (I = 4 H^2)
+//   5: r (V - X3) (0), S1 J (1)
+// The operations, subtraction constants and special cases are those of
+// jac26_add; every lane returns the same point.
+UFS_DEV void jac26_add_lanes(jac26& out, const jac26& a, const jac26& b) {
+    const int lane = (int)(threadIdx.x & 63u);
+    fe26 p, q, r1, r2, r3, r4, r5, z1z1, z2z2, u1, u2, s1, s2, h, rr, i4, j, v, t, t2, z;
+    // level 1
+    fe26_select(p, lane == 1, b.z, a.z);
+    fe26_select(q, lane == 0, a.z, b.z);
+    fe26_mul(r1, p, q);                 // Z1^2 | Z2^2 | Z1 Z2
+    fe26_shfl(z1z1, r1, 0);
+    fe26_shfl(z2z2, r1, 1);
+    // level 2
+    fe26_select(p, lane == 0, a.x, b.x);
+    fe26_select(p, lane == 2, b.z, p);
+    fe26_select(p, lane == 3, a.z, p);
+    fe26_select(q, (lane == 0) | (lane == 2), z2z2, z1z1);
+    fe26_mul(r2, p, q);                 // U1 | U2 | Z2^3 | Z1^3
+    fe26_shfl(u1, r2, 0);
+    fe26_shfl(u2, r2, 1);
+    fe26_sub<2>(t, u2, u1);
+    fe26_norm(h, t);                    // H = U2 - U1
+    // level 3
+    fe26_shfl(t, r2, 2);                // Z2^3
+    fe26_shfl(t2, r2, 3);               // Z1^3
+    fe26_select(q, lane == 0, t, t2);
+    fe26_select(q, lane == 2, h, q);
+    fe26_shfl(z, r1, 2);                // Z1 Z2
+    fe26_select(q, lane == 3, z, q);
+    fe26_select(p, lane == 0, a.y, b.y);
+    fe26_select(p, (lane == 2) | (lane == 3), h, p);
+    fe26_mul(r3, p, q);                 // S1 | S2 | H^2 | Z1 Z2 H
+    fe26_shfl(s1, r3, 0);
+    fe26_shfl(s2, r3, 1);
+    fe26_sub<2>(t, s2, s1);
+    fe26_add(t, t, t);
+    fe26_norm(rr, t);                   // r = 2(S2 - S1)
+    fe26_shfl(t, r3, 2);
+    fe26_shl<2>(i4, t);                 // I = (2H)^2
+    // level 4
+    fe26_select(p, lane == 1, u1, h);
+    fe26_select(p, lane == 2, rr, p);
+    fe26_select(q, lane == 2, rr, i4);
+    fe26_mul(r4, p, q);                 // J | V | r^2
+    fe26_shfl(j, r4, 0);
+    fe26_shfl(v, r4, 1);
+    fe26_shfl(t, r4, 2);
+    jac26 r;
+    fe26_sub<2>(t, t, j);
+    fe26_add(t2, v, v);
+    fe26_sub<3>(r.x, t, t2);            // X3 = r^2 - J - 2V
+    // level 5
+    fe26_sub<7>(t, v, r.x);
+    fe26_select(p, lane == 0, rr, s1);
+    fe26_select(q, lane == 0, t, j);
+    fe26_mul(r5, p, q);                 // r (V - X3) | S1 J
+    fe26_shfl(t, r5, 0);
+    fe26_shfl(j, r5, 1);
+    fe26_add(j, j, j);
+    fe26_sub<3>(r.y, t, j);             // Y3 = r (V - X3) - 2 S1 J
+    fe26_shfl(t, r3, 3);
+    fe26_add(r.z, t, t);                // Z3 = 2 Z1 Z2 H
+    r.inf = 0;
+    // the special cases of jac26_add (uniform: every lane holds the same)
+    if (a.inf | b.inf | fe26_normed_maybe_zero(h)) {
+        const bool degen = fe26_normed_is_zero(h) & !a.inf & !b.inf;
+        const bool same = fe26_normed_is_zero(rr);
+        if (degen & same) jac26_dbl(r, a);
+        if (degen & !same) jac26_set_inf(r);
+        if (b.inf) r = a;
+        if (a.inf) r = b;
+    }
+    out = r;
+}
+
 }  // namespace ufs

@@ -313,20 +313,22 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
 // kJoinSpan chunks after j, writes the bucket and empties the chain's slots;
 // longer chains (buckets holding a large share of the terms) stay for the
 // levels.  Each head belongs to one chain, so no slot is written by two
-// lanes; a lane reads the tail slot of its chain's last chunk, which that
-// chunk's own lane may empty meanwhile -- either value reads as "not kCont".
+// lanes.  The one slot two lanes touch is the tail of a chain's last chunk:
+// the chain's lane reads it while that chunk's own lane may empty it (its own
+// chain joined).  Both accesses are relaxed atomics, and either value reads as
+// "not kCont", so the result does not depend on the order.
 constexpr u32 kJoinSpan = 8;
 
 __global__ void __launch_bounds__(kMsmBlock)
 k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __restrict__ bsum) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= chunks) return;
-    const u32 tk = keys[2 * j + 1];
+    const u32 tk = __hip_atomic_load(&keys[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (tk >= kCont) return;
     u32 m = 1;
     for (; m <= kJoinSpan && j + m < chunks; ++m) {
         if (keys[2 * (j + m)] != tk) return;   // not a chain of this bucket: leave it
-        if (keys[2 * (j + m) + 1] != kCont) break;
+        if (__hip_atomic_load(&keys[2 * (j + m) + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kCont) break;
     }
     if (m > kJoinSpan || j + m >= chunks) return;
     jac26 acc, b;
@@ -338,7 +340,7 @@ k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __
     jac r;
     jac26_to_jac(r, acc);
     bsum[tk] = r;
-    keys[2 * j + 1] = kEmpty;
+    __hip_atomic_store(&keys[2 * j + 1], kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (u32 i = 1; i <= m; ++i) keys[2 * (j + i)] = kEmpty;
 }
 

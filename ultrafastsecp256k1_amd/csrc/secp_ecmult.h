@@ -243,6 +243,31 @@ struct gcomb {
     int nwin;
 };
 
+// Per-lane digit store in LDS for ecmult_dual: words [0, 4) hold the
+// variable-base digits of k1 and words [4, 8) those of k2 (4 bits each: table
+// index 0..7 and the sign, 8 digits per word), words [8, 16) the odd comb
+// scalar.  Word q of lane t sits at q * kDigitLanes + t, so a wave's reads hit
+// 64 consecutive words.  The scalars leave the registers before the step loop:
+// recoded once with constant bit positions instead of one dynamic bit-field
+// selection per window, with nothing of them to spill across the loop.
+constexpr int kDigitLanes = 256;   // the largest block that calls ecmult_dual
+constexpr int kDigitWords = 16;
+
+UFS_DEV u32* lane_digit_store() {
+    __shared__ u32 store[kDigitWords * kDigitLanes];
+    return store + threadIdx.x;
+}
+
+// Bit field [pos, pos + w) of the 256-bit value in words base.. of the store.
+UFS_DEV u32 lds_bits_at(const u32* st, int base, int pos, int w) {
+    const int wi = pos >> 5, sh = pos & 31;
+    const u32 lo = wi < 8 ? st[(base + wi) * kDigitLanes] : 0u;
+    const u32 hi = wi + 1 < 8 ? st[(base + wi + 1) * kDigitLanes] : 0u;
+    const u64 v = ((u64)hi << 32) | lo;
+    return (u32)(v >> sh) & ((1u << w) - 1u);
+}
+
+
 // ---------------------------------------------------------------------------
 // Double-scalar multiplication u*G + (k1 + lambda k2)*P' as one flat loop of
 // steps, so a single inline copy of the doubling and of the mixed addition
@@ -282,6 +307,27 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
     const int half = 1 << (G.w - 1);
     constexpr int kVarSteps = kPWindows * (kPW + 2);
     const int nsteps = kVarSteps + 3 + 1 + G.nwin;
+    u32* const dig = lane_digit_store();
+    {
+        static_assert(kPW == 4 && kPWindows % 8 == 0, "4-bit digits, 8 per word");
+#pragma unroll
+        for (int q = 0; q < kPWindows / 8; ++q) {
+            u32 w1 = 0, w2 = 0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                u32 idx;
+                bool ng;
+                odd_digit(k1.v, 8 * q + e, kPW, idx, ng);
+                w1 |= (idx | (ng ? 8u : 0u)) << (4 * e);
+                odd_digit(k2.v, 8 * q + e, kPW, idx, ng);
+                w2 |= (idx | (ng ? 8u : 0u)) << (4 * e);
+            }
+            dig[q * kDigitLanes] = w1;
+            dig[(4 + q) * kDigitLanes] = w2;
+        }
+#pragma unroll
+        for (int l = 0; l < 8; ++l) dig[(8 + l) * kDigitLanes] = ku.v[l];
+    }
 
     // accumulator (10 x 26-bit limbs) starts at the top digits,
     // P' + phi(P') (both 1: k < 2^128): P' itself here, phi(P') as the first
@@ -327,16 +373,11 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
                 fl = f2;
             } else if (step < kVarSteps) {
                 const int w = kPWindows - 1 - step / (kPW + 2);
-                u32 idx;
-                bool ng;
-                if (r == kPW) {
-                    odd_digit(k1.v, w, kPW, idx, ng);
-                    fl = ng ? (f1 ^ kAddNeg) : f1;
-                } else {
-                    odd_digit(k2.v, w, kPW, idx, ng);
-                    fl = ng ? (f2 ^ kAddNeg) : f2;
-                }
-                bp = slot + idx;
+                const bool second = r != kPW;
+                const u32 nib = (dig[((second ? 4 : 0) + (w >> 3)) * kDigitLanes] >> (4 * (w & 7))) & 15u;
+                const u32 f = second ? f2 : f1;
+                fl = (nib & 8u) ? (f ^ kAddNeg) : f;
+                bp = slot + (nib & 7u);
             } else if (step < kVarSteps + 2) {
                 const bool first = step == kVarSteps;
                 bp = slot;
@@ -347,11 +388,12 @@ UFS_DEV void ecmult_dual(jac& out, const aff* slot, sc k1, bool neg1, sc k2, boo
                 fl = fg;
             } else {
                 const int i = step - (kVarSteps + 4);
-                u32 idx;
-                bool ng;
-                odd_digit(ku.v, i, G.w, idx, ng);
-                bp = G.tab + (size_t)i * half + idx;
-                fl = ng ? (fg ^ kAddNeg) : fg;
+                // odd_digit on the stored comb scalar
+                const u32 win = lds_bits_at(dig, 8, G.w * i, G.w) | 1u;
+                const u32 nb = lds_bits_at(dig, 8, G.w * (i + 1), 1);
+                const u32 mag = nb ? win : ((1u << G.w) - win);
+                bp = G.tab + (size_t)i * half + ((mag - 1u) >> 1);
+                fl = nb ? fg : (fg ^ kAddNeg);
             }
             jac26 t;
             bool nd;

@@ -252,3 +252,31 @@ def test_full_size_batch_one_launch_matches_reference(gpu, golden, kind):
     fn = gpu.ecdsa_verify_batch if kind == "ecdsa" else gpu.schnorr_verify_batch
     v2 = fn(b.msgs, b.pubkeys, b.sigs)
     assert hashlib.sha256(v2.tobytes()).hexdigest() == f["sha256_verdicts"]
+
+
+@pytest.mark.parametrize("w", [4, 9, 13, 19, 20])
+def test_comb_widths_agree(golden, monkeypatch, w):
+    """The fixed-base comb for G at other widths (the tuning key gcomb_w,
+    overridden by UFSECP_GCOMB_W): 4 to 20 bits, with window counts that do
+    not divide 256 evenly, give the reference verdicts for both schemes, the
+    edge cases and the Taproot commitments, and the reference's recovered
+    keys."""
+    from ultrafastsecp256k1_amd import UfsecpGpu
+    monkeypatch.setenv("UFSECP_GCOMB_W", str(w))
+    with UfsecpGpu(0) as g:
+        for kind in ("ecdsa", "schnorr"):
+            f = golden["workload_heads"][kind]
+            b = wl.generate(kind, f["n"], f["seed"])
+            fn = g.ecdsa_verify_batch if kind == "ecdsa" else g.schnorr_verify_batch
+            v = fn(b.msgs, b.pubkeys, b.sigs)
+            assert hashlib.sha256(v.tobytes()).hexdigest() == f["sha256_verdicts"], (w, kind)
+            cases = golden["edge_cases"][kind]
+            v = fn(_cat(cases, "msg"), _cat(cases, "pub"), _cat(cases, "sig"))
+            assert [int(x) for x in v] == [c["verdict"] for c in cases], (w, kind)
+        d = golden["pubdata_vectors"]["commitment"]
+        par = np.array([c["parity"] for c in d], np.uint8)
+        v = g.commitment_verify(_cat(d, "internal"), _cat(d, "tweak"), _cat(d, "tweaked"), par)
+        assert [int(x) for x in v] == [c["verdict"] for c in d], w
+        d = golden["recover_vectors"]
+        keys, valid = g.ecrecover_batch(_cat(d, "msg"), _cat(d, "sig"), [c["recid"] for c in d])
+        assert [(int(a), k.tobytes().hex()) for a, k in zip(valid, keys)] == [(c["valid"], c["key"]) for c in d], w

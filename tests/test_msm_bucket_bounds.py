@@ -1,12 +1,14 @@
 """CPU replay of the MSM's term preparation and bucket-entry arithmetic
-(ultrafastsecp256k1_amd/csrc/msm.hip: the GLV split and masking of
-k_msm_prepare, msm_digit, k_msm_entries) for adversarial scalars and every
-window width c = 2..20.
+(ultrafastsecp256k1_amd/csrc/msm.hip: the GLV split, masking and digits of
+k_msm_entries and msm_digit) for adversarial scalars and every window width
+c = 2..20.
 
 The accumulation kernel indexes bsum[bucket id] and pts[term index] with the
 entries' fields, so these must stay below the bucket count (W + 1) * 2^(c-1)
 and 2N for any bytes a caller passes: scalars >= n, 2^256 - 1, zero, halves at
-the 2^128 edge, invalid points.  The replay follows the device code step by
+the 2^128 edge.  Invalid points do not change the entries (their terms keep
+their buckets and the call returns the error), so only the scalars matter
+here.  The replay follows the device code step by
 step (same constants, one conditional subtraction in sc_add, round-half-up of
 k * g / 2^384), and the closed forms check it: k1 + lambda k2 == k (mod n) and
 sum_w d_w 2^(c w) == the half, so a replay that drifted from the kernel would
@@ -62,11 +64,11 @@ def glv_split_dev(k):
     return k1, k2, neg1, neg2
 
 
-def prepare(k, point_ok=True):
-    """k_msm_prepare: halves cut to 128 bits; a failing term (scalar >= n or
-    bad point) gets zero halves, so it places only skip entries."""
+def prepare(k):
+    """k_msm_entries: halves cut to 128 bits; a scalar >= n gets zero halves,
+    so it places only skip entries."""
     k1, k2, n1, n2 = glv_split_dev(k)
-    if not (k < N and point_ok):
+    if not k < N:
         return 0, 0
     return k1 & ((1 << 128) - 1), k2 & ((1 << 128) - 1)
 
@@ -145,21 +147,20 @@ def test_digits_recompose_every_window_width(c):
 
 @pytest.mark.parametrize("c", range(2, 21))
 def test_bucket_ids_in_range_for_any_input(c):
-    """Prepared terms of adversarial scalars and points, and (without the
-    masking) raw 256-bit values: every id < (W + 1) B, every skip id inside
+    """Split terms of adversarial scalars, and (without the masking) raw
+    256-bit values: every id < (W + 1) B, every skip id inside
     its own window, every term index < 2N."""
     W, B = plan(c)
     nbk = (W + 1) * B
     scalars = adversarial_scalars()
     n2 = 2 * len(scalars)
     for t, k in enumerate(scalars):
-        for ok in (True, False):
-            for h, half in enumerate(prepare(k, ok)):
-                i = 2 * t + h
-                assert i < n2
-                for w, (bid, skip) in enumerate(entries(half, c, i)):
-                    assert bid < nbk, (hex(k), c, w)
-                    assert w * B <= bid < (w + 2 if w == W - 1 else w + 1) * B
+        for h, half in enumerate(prepare(k)):
+            i = 2 * t + h
+            assert i < n2
+            for w, (bid, skip) in enumerate(entries(half, c, i)):
+                assert bid < nbk, (hex(k), c, w)
+                assert w * B <= bid < (w + 2 if w == W - 1 else w + 1) * B
     rnd = random.Random(100 + c)
     for v in [M256, 1 << 255, (1 << 256) - (1 << 128)] + [rnd.getrandbits(256) for _ in range(64)]:
         for bid, _ in entries(v, c, 5):

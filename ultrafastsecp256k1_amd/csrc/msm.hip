@@ -64,10 +64,11 @@ UFS_DEV int msm_digit(const u32 k[8], int w, int c, int W, u32& carry) {
 // term becomes two of 128 bits, on P and phi(P) = (beta x, y) = lambda P.
 constexpr int kScalarBits = 128;
 
-// Parse + decompress every term (scalar < n, strict SEC1 point; the first
-// failing term decides the error, reported as 2i or 2i+1 through atomicMin),
-// and split it into the two GLV terms 2i: (|k1|, +-P) and 2i+1: (|k2|, +-phi(P)).
-constexpr int kPrepTermsPerThread = 4;
+// Bases: one term per lane, a grid of at most kPointBlocksPerCu blocks per CU
+// striding over the terms.
+constexpr int kPointBlocksPerCu = 16;
+// Scalars: one term per lane, likewise.
+constexpr int kEntryBlocksPerCu = 16;
 
 // A base point given as 64 bytes x || y (big-endian affine coordinates):
 // valid when x, y < p and y^2 == x^3 + 7 (two squarings and a
@@ -96,71 +97,43 @@ UFS_DEV bool load_affine64(aff& P, const u8* pp) {
     return lt & fe_equal(lhs, rhs);
 }
 
-// AFFINE: points are 64-byte x || y (pre-decompressed bases), else 33-byte
-// SEC1 compressed.
+// Bases of the MSM: every point is parsed (strict SEC1 compressed, or
+// AFFINE: 64-byte x || y checked on the curve) and stored as P at pts[2i] and
+// phi(P) = (beta x, y) = lambda P at pts[2i + 1], the bases of the two GLV
+// halves of term i.  The halves' signs are applied by their bucket entries
+// (k_msm_entries), so this kernel does not read the scalars: it runs on the
+// engine's second stream while the scalars are split and sorted.  A failing
+// point reports 2i + 1 through atomicMin (scalars report 2i), so the smallest
+// failing term decides the error.  Its entries stay in the buckets with the
+// unchecked coordinates, which only costs arithmetic: the call then returns
+// the error and discards the sums.
 template <bool AFFINE>
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_prepare(const u8* __restrict__ scalars32, const u8* __restrict__ points, u32 n, aff* __restrict__ pts,
-              sc* __restrict__ scl, u32* __restrict__ err) {
-    const size_t step = (size_t)gridDim.x * blockDim.x * kPrepTermsPerThread;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x * kPrepTermsPerThread; base < n; base += step) {
-        for (int j = 0; j < kPrepTermsPerThread; ++j) {
-            const size_t i = base + (size_t)j * blockDim.x + threadIdx.x;
-            if (i >= n) break;
-            u32 k[8];
-            msm_load_scalar(k, scalars32 + 32ull * i);
-            const bool sok = u256_lt_n(k);
-            aff P;
-            bool pok;
-            if (AFFINE) {
-                pok = load_affine64(P, points + 64ull * i);
-            } else {
-                const u8* pp = points + 33ull * i;
-                const u32 prefix = pp[0];
-                u32 x[8];
+k_msm_points(const u8* __restrict__ points, u32 n, aff* __restrict__ pts, u32* __restrict__ err) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        aff P;
+        bool pok;
+        if (AFFINE) {
+            pok = load_affine64(P, points + 64ull * i);
+        } else {
+            const u8* pp = points + 33ull * i;
+            const u32 prefix = pp[0];
+            u32 x[8];
 #pragma unroll
-                for (int l = 0; l < 8; ++l) {
-                    const u8* q = pp + 1 + 28 - 4 * l;
-                    x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
-                }
-                pok = lift_x(P, x, prefix & 1u);
-                pok &= (prefix == 2u) | (prefix == 3u);
+            for (int l = 0; l < 8; ++l) {
+                const u8* q = pp + 1 + 28 - 4 * l;
+                x[l] = ((u32)q[0] << 24) | ((u32)q[1] << 16) | ((u32)q[2] << 8) | (u32)q[3];
             }
-            if (!sok) {
-                atomicMin(err, 2u * (u32)i);
-            } else if (!pok) {
-                atomicMin(err, 2u * (u32)i + 1u);
-            }
-            sc s, k1, k2;
-            bool n1, n2;
-#pragma unroll
-            for (int l = 0; l < 8; ++l) s.v[l] = k[l];
-            glv_split(k1, k2, n1, n2, s);
-            // The halves are below 2^128 for every scalar below n.  A term
-            // that fails its checks (the call then returns the error) places
-            // no bucket entries: a scalar >= n would split into halves of any
-            // size, whose top-window digits would leave the bucket range.
-#pragma unroll
-            for (int l = 4; l < 8; ++l) {
-                k1.v[l] = 0;
-                k2.v[l] = 0;
-            }
-            if (!(sok && pok)) {
-                sc_set_u32(k1, 0);
-                sc_set_u32(k2, 0);
-            }
-            aff Q;
-            const fe BETA = {{UFS_BETA_LIMBS}};
-            fe_mul(Q.x, P.x, BETA);
-            fe ny;
-            fe_neg(ny, P.y);
-            Q.y = n2 ? ny : P.y;
-            if (n1) P.y = ny;
-            pts[2 * i] = P;
-            pts[2 * i + 1] = Q;
-            scl[2 * i] = k1;
-            scl[2 * i + 1] = k2;
+            pok = lift_x(P, x, prefix & 1u);
+            pok &= (prefix == 2u) | (prefix == 3u);
         }
+        if (!pok) atomicMin(err, 2u * i + 1u);
+        aff Q;
+        const fe BETA = {{UFS_BETA_LIMBS}};
+        fe_mul(Q.x, P.x, BETA);
+        Q.y = P.y;
+        pts[2 * i] = P;
+        pts[2 * i + 1] = Q;
     }
 }
 
@@ -175,10 +148,34 @@ UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg, bool skip) {
     return ((u64)bucket << 32) | (u64)(term | (neg ? kEntrySign : 0u) | (skip ? kEntrySkip : 0u));
 }
 
-// Unsorted entries in window-major order (one coalesced 8-byte store per
-// (half-term, window)).  A zero digit becomes a skip entry in one of its own
-// window's buckets (chosen by the term index, so skips spread evenly and never
-// form long runs), which the accumulation passes over.
+// The W entries of half-term i (value v below 2^128, sign flip hneg): one
+// per window, window-major, zero digits as skip entries.
+UFS_DEV void msm_half_entries(const u32 v[8], bool hneg, u32 i, u32 n2, u64* __restrict__ entries, int c, int W,
+                              u32* __restrict__ range_err) {
+    const u32 B = 1u << (c - 1);
+    u32 carry = 0;
+    for (int w = 0; w < W; ++w) {
+        const int d = msm_digit(v, w, c, W, carry);
+        const u32 mag = (u32)(d < 0 ? -d : d);
+        const u32 lim = w == W - 1 ? 2u * B : B;
+        const bool out = mag > lim;
+        if (out) atomicOr(range_err, 1u);
+        const bool skip = d == 0 || out;
+        const u32 b = skip ? (i & (B - 1u)) : mag - 1u;
+        entries[(size_t)w * n2 + i] = msm_entry((u32)w * B + b, i, (d < 0) != hneg, skip);
+    }
+}
+
+// Scalars -> unsorted bucket entries, one term per lane: the scalar is
+// checked (< n, else 2i into err[0] through atomicMin and no entries but
+// skips), split by GLV into k = +-k1 +- lambda k2 with halves below 2^128, and
+// each half's signed c-bit digits become its entries: half-term 2i on P,
+// 2i + 1 on phi(P), an entry's sign the digit's sign flipped by its half's.
+//
+// Entries are window-major (window w's entries at [w 2n, (w + 1) 2n)).  A
+// zero digit becomes a skip entry in one of its own window's buckets (chosen
+// by the half-term index, so skips spread evenly and never form long runs),
+// which the accumulation passes over.
 //
 // Order: the sort keys on the low c bits of the bucket id only, which are
 // distinct for distinct buckets of one window ((w & 1) * B + b below the top
@@ -189,27 +186,37 @@ UFS_DEV u64 msm_entry(u32 bucket, u32 term, bool neg, bool skip) {
 // log2 of the bucket count (three passes).
 //
 // Bound: window w < W - 1 owns ids [wB, (w + 1)B), the top window [(W - 1)B,
-// (W + 1)B).  Prepared halves are below 2^128, so every digit fits (checked on
-// the CPU by replaying this arithmetic for any input); a digit that would
-// not becomes a skip entry and raises *range_err, which the final kernel
-// reports as an internal error, so no later kernel indexes outside bsum.
+// (W + 1)B).  The halves are cut to 128 bits (a scalar >= n would split into
+// halves of any size) and are below 2^128 for every scalar below n, so every
+// digit fits (checked on the CPU by replaying this arithmetic for any input);
+// a digit that would not becomes a skip entry and raises err[1], which the
+// final kernel reports as an internal error, so no later kernel indexes
+// outside bsum.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_entries(const sc* __restrict__ scl, u32 n2, u64* __restrict__ entries, int c, int W,
-              u32* __restrict__ range_err) {
-    const u32 B = 1u << (c - 1);
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
-        const sc s = scl[i];
-        u32 carry = 0;
-        for (int w = 0; w < W; ++w) {
-            const int d = msm_digit(s.v, w, c, W, carry);
-            const u32 mag = (u32)(d < 0 ? -d : d);
-            const u32 lim = w == W - 1 ? 2u * B : B;
-            const bool out = mag > lim;
-            if (out) atomicOr(range_err, 1u);
-            const bool skip = d == 0 || out;
-            const u32 b = skip ? (i & (B - 1u)) : mag - 1u;
-            entries[(size_t)w * n2 + i] = msm_entry((u32)w * B + b, i, d < 0, skip);
+k_msm_entries(const u8* __restrict__ scalars32, u32 n, u64* __restrict__ entries, int c, int W,
+              u32* __restrict__ err) {
+    const u32 n2 = 2 * n;
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        u32 k[8];
+        msm_load_scalar(k, scalars32 + 32ull * t);
+        const bool sok = u256_lt_n(k);
+        if (!sok) atomicMin(err, 2u * t);
+        sc s, k1, k2;
+        bool n1, n2h;
+#pragma unroll
+        for (int l = 0; l < 8; ++l) s.v[l] = k[l];
+        glv_split(k1, k2, n1, n2h, s);
+#pragma unroll
+        for (int l = 4; l < 8; ++l) {
+            k1.v[l] = 0;
+            k2.v[l] = 0;
         }
+        if (!sok) {
+            sc_set_u32(k1, 0);
+            sc_set_u32(k2, 0);
+        }
+        msm_half_entries(k1.v, n1, 2 * t, n2, entries, c, W, err + 1);
+        msm_half_entries(k2.v, n2h, 2 * t + 1, n2, entries, c, W, err + 1);
     }
 }
 
@@ -581,16 +588,15 @@ hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, bool affine, uint32_t n, void* pts,
-                              void* scl, uint32_t* err, unsigned max_blocks, hipStream_t st) {
-    unsigned g = blocks_for(n, (unsigned)kMsmBlock * kPrepTermsPerThread);
-    if (g > max_blocks) g = max_blocks;
+hipError_t launch_msm_points(const uint8_t* points, bool affine, uint32_t n, void* pts, uint32_t* err,
+                             unsigned cus, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    unsigned g = blocks_for(n, kMsmBlock);
+    if (g > cus * kPointBlocksPerCu) g = cus * kPointBlocksPerCu;
     if (affine)
-        hipLaunchKernelGGL(k_msm_prepare<true>, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points, n, (aff*)pts,
-                           (sc*)scl, err);
+        hipLaunchKernelGGL(k_msm_points<true>, dim3(g), dim3(kMsmBlock), 0, st, points, n, (aff*)pts, err);
     else
-        hipLaunchKernelGGL(k_msm_prepare<false>, dim3(g), dim3(kMsmBlock), 0, st, scalars32, points, n, (aff*)pts,
-                           (sc*)scl, err);
+        hipLaunchKernelGGL(k_msm_points<false>, dim3(g), dim3(kMsmBlock), 0, st, points, n, (aff*)pts, err);
     return hipGetLastError();
 }
 
@@ -694,12 +700,14 @@ GpuError MsmEngine::init() {
     return GpuError::Ok;
 }
 
-// The split form's second stream (default priority) and its two events.
+// The second stream (default priority) and its events: the bases'
+// decompression and the split form's high windows run on it.
 hipError_t MsmEngine::ensure_aux() {
     hipError_t e = hipSuccess;
     if (!aux_) e = hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking);
     if (e == hipSuccess && !fork_) e = hipEventCreateWithFlags(&fork_, hipEventDisableTiming);
     if (e == hipSuccess && !join_) e = hipEventCreateWithFlags(&join_, hipEventDisableTiming);
+    if (e == hipSuccess && !ready_) e = hipEventCreateWithFlags(&ready_, hipEventDisableTiming);
     return e;
 }
 
@@ -765,7 +773,8 @@ MsmEngine::~MsmEngine() {
     }
     if (fork_) (void)hipEventDestroy(fork_);
     if (join_) (void)hipEventDestroy(join_);
-    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &scl_, &entries_,
+    if (ready_) (void)hipEventDestroy(ready_);
+    DeviceBuffer* all[] = {&in_s_, &in_p_, &pts_, &entries_,
                            &bsum_, &seg_, &red_a_, &red_b_, &err_, &out_, &slot_keys_[0],
                            &slot_keys_[1], &slot_part_[0], &slot_part_[1],
                            &rlc_in_, &rlc_terms_, &rlc_part_, &sorted_, &sort_tmp_};
@@ -804,7 +813,7 @@ GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLa
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM temporary-storage size query", e);
     L.sort_bytes = sort_bytes;
     const size_t sizes[MsmLayout::kParts] = {
-        2 * n * sizeof(aff), 2 * n * sizeof(sc), L.max_entries * 8, L.max_entries * 8, nbk * jsz, nseg * jsz,
+        2 * n * sizeof(aff), L.max_entries * 8, L.max_entries * 8, nbk * jsz, nseg * jsz,
         nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S * 4, S * jsz, S * jsz, L.sort_bytes + 16};
     size_t off = 0;
     for (int i = 0; i < MsmLayout::kParts; ++i) {
@@ -823,7 +832,7 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
     GpuError r = msm_layout(n, p, st, L);
     if (r != GpuError::Ok) return r;
     // the engine's own grow-only buffers, one per part of the layout
-    DeviceBuffer* bufs[MsmLayout::kParts] = {&pts_, &scl_, &entries_, &sorted_, &bsum_, &seg_, &red_a_, &red_b_,
+    DeviceBuffer* bufs[MsmLayout::kParts] = {&pts_, &entries_, &sorted_, &bsum_, &seg_, &red_a_, &red_b_,
                                               &err_, &slot_keys_[0], &slot_keys_[1], &slot_part_[0], &slot_part_[1],
                                               &sort_tmp_};
     void* ptr[MsmLayout::kParts];
@@ -841,7 +850,7 @@ GpuError MsmEngine::enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_po
 
 namespace {
 // the parts of MsmLayout, in order
-enum MsmPart { kPts, kScl, kEntries, kSorted, kBsum, kSeg, kRedA, kRedB, kErr, kKeys0, kKeys1, kPart0, kPart1,
+enum MsmPart { kPts, kEntries, kSorted, kBsum, kSeg, kRedA, kRedB, kErr, kKeys0, kKeys1, kPart0, kPart1,
                kSortTmp };
 // the high windows' share sits after the two error words in the kErr part
 constexpr size_t kPartOffset = 64;
@@ -850,9 +859,11 @@ constexpr size_t kPartOffset = 64;
 // Buckets of windows [wa, wb): their entries are the window-major range
 // [wa 2n, wb 2n) of the unsorted list; they are sorted into the same range,
 // accumulated by chunks (level 0), joins and levels into bsum.  The slot and
-// sort buffers are reused by the next call on the same stream.
+// sort buffers are reused by the next call on the same stream.  With
+// `points` set, st waits for that event (the bases) between the sort and the
+// accumulation.
 GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* const* ptr, size_t n, int wa, int wb,
-                                 hipStream_t st) {
+                                 hipEvent_t points, hipStream_t st) {
     const size_t n2 = 2 * n;
     const size_t off = (size_t)wa * n2, E = (size_t)(wb - wa) * n2;
     // k_seg0 runs two waves per SIMD: 512 lanes per CU.  Chunks of 64 halve
@@ -874,6 +885,8 @@ GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* con
         e = hipcub::DeviceRadixSort::SortKeys(ptr[kSortTmp], sort_bytes, entries, sorted, (int)E, 32, 32 + p.c, st);
         if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket sort", e);
     }
+    if (points && (e = hipStreamWaitEvent(st, points, 0)) != hipSuccess)
+        return be_->fail(GpuError::Backend, "MSM wait for the bases", e);
     hipLaunchKernelGGL(k_seg0, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, (const aff*)ptr[kPts],
                        (const uint64_t*)sorted, (uint32_t)E, K, (uint32_t)chunks0, bsum, slot_keys[0],
                        slot_part[0]);
@@ -945,22 +958,45 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
     const size_t nbk = (size_t)p.groups * p.buckets;
     hipError_t e;
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
-    const unsigned prep_blocks = cus * 4;
     uint32_t* err = (uint32_t*)ptr[kErr];
     void* part = (uint8_t*)ptr[kErr] + kPartOffset;
     jac* bsum = (jac*)ptr[kBsum];
 
     // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
-    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess || (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess ||
-        (e = launch_msm_prepare(d_scalars32, d_points, affine, (uint32_t)n, ptr[kPts], ptr[kScl], err, prep_blocks,
-                                st)) !=
-            hipSuccess)
-        return be_->fail(GpuError::Launch, "MSM prepare", e);
+    if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
+        (e = launch_u32_fill(err + 1, 0u, 1, st)) != hipSuccess)
+        return be_->fail(GpuError::Launch, "MSM error words", e);
+    // The bases are decompressed on the second stream (integer VALU work)
+    // while st splits the scalars into bucket entries and sorts them (memory
+    // and latency work); st waits for the bases just before the first
+    // accumulation.  Without the second stream the two run in turn on st.
+    hipStream_t ps = st;
+    if (aux_ && ready_ && hipEventRecord(fork_, st) == hipSuccess && hipStreamWaitEvent(aux_, fork_, 0) == hipSuccess)
+        ps = aux_;
+    if ((e = launch_msm_points(d_points, affine, (uint32_t)n, ptr[kPts], err, cus, ps)) == hipSuccess && ps != st)
+        e = hipEventRecord(ready_, ps);
+    if (e != hipSuccess) {
+        if (ps != st) (void)hipStreamSynchronize(ps);
+        return be_->fail(GpuError::Launch, "MSM bases", e);
+    }
+    const hipEvent_t points = ps != st ? ready_ : nullptr;
+    // every return from here leaves st waiting for the bases (after the
+    // bucket phase's own wait this adds nothing), so no later work on st
+    // or on another stream ordered after st's completion event runs beside
+    // the decompression
+    struct PointsJoin {
+        hipStream_t st;
+        hipEvent_t ev;
+        ~PointsJoin() {
+            if (ev) (void)hipStreamWaitEvent(st, ev, 0);
+        }
+    } join_points{st, points};
     // unsorted window-major entries of every window (zero digits are skip entries)
     {
-        const uint32_t n2 = (uint32_t)(2 * n);
-        hipLaunchKernelGGL(k_msm_entries, dim3(blocks_for(n2, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const sc*)ptr[kScl], n2, (uint64_t*)ptr[kEntries], p.c, p.windows, err + 1);
+        unsigned g = blocks_for(n, kMsmBlock);
+        if (g > cus * kEntryBlocksPerCu) g = cus * kEntryBlocksPerCu;
+        hipLaunchKernelGGL(k_msm_entries, dim3(g), dim3(kMsmBlock), 0, st, d_scalars32, (uint32_t)n,
+                           (uint64_t*)ptr[kEntries], p.c, p.windows, err);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
     }
     // empty buckets stay at infinity
@@ -970,7 +1006,7 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
     const void* rows = nullptr;
     GpuError r;
     if (!split) {
-        if ((r = bucket_phase(L, p, ptr, n, 0, p.windows, st)) != GpuError::Ok) return r;
+        if ((r = bucket_phase(L, p, ptr, n, 0, p.windows, points, st)) != GpuError::Ok) return r;
         if ((r = window_phase(p, ptr, 0, p.windows, st, &rows)) != GpuError::Ok) return r;
         e = launch_msm_final(rows, (uint32_t)p.windows, true, (uint32_t)p.c, 0, nullptr, err, d_out33, d_status,
                              nullptr, st);
@@ -978,7 +1014,7 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
         return GpuError::Ok;
     }
     const int ws = p.windows / 2;
-    if ((r = bucket_phase(L, p, ptr, n, ws, p.windows, st)) != GpuError::Ok) return r;
+    if ((r = bucket_phase(L, p, ptr, n, ws, p.windows, points, st)) != GpuError::Ok) return r;
     if ((e = hipEventRecord(fork_, st)) != hipSuccess || (e = hipStreamWaitEvent(aux_, fork_, 0)) != hipSuccess)
         return be_->fail(GpuError::Backend, "MSM fork", e);
     // the high windows' share, on the second stream
@@ -993,7 +1029,7 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
     // the low windows on st; st then waits for the share (also when a launch
     // failed, so that nothing later on st runs beside the second stream)
     r = rh;
-    if (r == GpuError::Ok) r = bucket_phase(L, p, ptr, n, 0, ws, st);
+    if (r == GpuError::Ok) r = bucket_phase(L, p, ptr, n, 0, ws, nullptr, st);
     if (r == GpuError::Ok) r = window_phase(p, ptr, 0, ws, st, &rows);
     if ((e = hipStreamWaitEvent(st, join_, 0)) != hipSuccess) return be_->fail(GpuError::Backend, "MSM join", e);
     if (r != GpuError::Ok) return r;

@@ -4,11 +4,12 @@
 // runs one full GLV scalar multiplication per term followed by a block tree
 // and a single-thread serial sum.  This engine is a bucket method with a
 // device-side sort by bucket instead:
-//   1. parse + decompress: scalars < n and strict SEC1 points checked per term
+//   1. parse + decompress the bases into P and phi(P) = (beta x, y), strict
+//      SEC1 points checked per term, on a second stream beside step 2
 //      (first failing term decides the error, as in the reference CPU
 //      ufsecp_multi_scalar_mul, impl/ufsecp_taproot.cpp:957-991);
-//   2. GLV split of every scalar, k = k1 + lambda k2 with |k1|, |k2| < 2^128,
-//      into the terms (|k1|, +-P) and (|k2|, +-phi(P)), phi(P) = (beta x, y);
+//   2. scalars < n checked and split by GLV, k = +-k1 +- lambda k2 with
+//      k1, k2 < 2^128, into the terms (k1, +-P) and (k2, +-phi(P));
 //      c-bit digits of the 128-bit halves in W = ceil(128/c) windows: signed,
 //      |d| <= 2^(c-1), below the top window, unsigned (bits + carry, up to
 //      2^c) in the top one; one 8-byte entry (bucket id, term index | sign)
@@ -49,8 +50,9 @@ int msm_default_window(size_t n);
 MsmPlan msm_plan(size_t n, int c, const Tuning* tun = nullptr);
 
 // Device launchers (msm.hip)
-hipError_t launch_msm_prepare(const uint8_t* scalars32, const uint8_t* points, bool affine, uint32_t n, void* pts,
-                              void* scl, uint32_t* err, unsigned max_blocks, hipStream_t st);
+// bases (33-byte compressed, or 64-byte x || y when affine) -> P, phi(P)
+hipError_t launch_msm_points(const uint8_t* points, bool affine, uint32_t n, void* pts, uint32_t* err,
+                             unsigned cus, hipStream_t st);
 // window sums of the bucket groups w0 .. w0 + gn - 1 into seg[0 .. gn * T)
 hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, uint32_t w0, uint32_t gn,
                                hipStream_t st);
@@ -63,7 +65,7 @@ hipError_t launch_u32_fill(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 // Device buffers of one MSM, as parts of one byte range: offsets are
 // 256-byte aligned; total is the range's size.
 struct MsmLayout {
-    static constexpr int kParts = 14;
+    static constexpr int kParts = 13;
     size_t off[kParts], size[kParts];
     size_t total, max_entries, chunks0, slots0, sort_bytes;
 };
@@ -107,7 +109,7 @@ private:
                      bool split, hipStream_t st);
     // Sort, accumulate and reduce the buckets of windows [wa, wb) into bsum.
     GpuError bucket_phase(const MsmLayout& L, const MsmPlan& p, void* const* ptr, size_t n, int wa, int wb,
-                          hipStream_t st);
+                          hipEvent_t points, hipStream_t st);
     // Window sums of windows [wa, wb) (and the top window's second bucket
     // group when wb is the last window); *rows receives one point per group.
     GpuError window_phase(const MsmPlan& p, void* const* ptr, int wa, int wb, hipStream_t st, const void** rows);
@@ -121,12 +123,13 @@ private:
     hipError_t finish(hipStream_t st);
     RocmBackend* be_;
     hipEvent_t done_ = nullptr;
-    // second stream of the split form, with its fork and join events
+    // second stream (the bases' decompression, the split form's high
+    // windows) with its fork and join events; ready_ marks the bases done
     hipStream_t aux_ = nullptr;
-    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr, ready_ = nullptr;
     hipStream_t last_stream_ = nullptr;
     bool pending_ = false;
-    DeviceBuffer in_s_, in_p_, pts_, scl_, entries_, bsum_, seg_, red_a_, red_b_, err_,
+    DeviceBuffer in_s_, in_p_, pts_, entries_, bsum_, seg_, red_a_, red_b_, err_,
         out_, slot_keys_[2], slot_part_[2];
     DeviceBuffer rlc_in_, rlc_terms_, rlc_part_, sorted_, sort_tmp_;
 };

@@ -153,75 +153,86 @@ UFS_DEV fe* slot_zr(aff* slot, int j) {
     return z + (j - 1);
 }
 
+// The odd multiples by co-Z arithmetic (Meloni's addition of two points
+// sharing Z; the table scheme of Longa and Miri; the formulas as XYCZ-ADD in
+// Goundar, Joye, Miyaji, Rivain and Venelli, "Scalar multiplication on
+// Weierstrass elliptic curves from Co-Z arithmetic", 2011).  A doubling of
+// the affine P gives Q = 2P and P itself at the same Z = 2y; each step then
+// adds Q to the last odd multiple, and the sum and Q come out sharing the
+// new Z = Z (x_last - x_Q), whose factor (x_last - x_Q) is the step's Z
+// ratio.  4M + 2S per step, against 7M + 4S for a mixed addition with a Z
+// update.  No step can meet Q = +-(2j-1)P: that needs (2j+1)P or (2j-3)P to
+// be infinity, impossible for a point of prime order n.
 UFS_DEV void build_ptable(aff* slot, const aff& P) {
-    jac Pj, D;
-    jac_from_aff(Pj, P);
-    jac_dbl(D, Pj);
-    // Isomorphic curve scaled by D.z: D becomes affine (D.x, D.y).
-    fe z2, z3;
-    fe_sqr(z2, D.z);
-    fe_mul(z3, z2, D.z);
-    fe cx, cy, cz;
-    fe_mul(cx, P.x, z2);
-    fe_mul(cy, P.y, z3);
-    fe_set_u32(cz, 1);
+    // from Z = 1: B = x^2, E = y^2, L = E^2, S = 2((x + E)^2 - B - L) = 4xy^2,
+    // M = 3B; 2P = (M^2 - 2S, M(S - X(2P)) - 8L) and P = (S, 8L), Z = 2y
+    fe b, e, l, t, qx, qy, cx, cy;
+    fe_sqr(b, P.x);
+    fe_sqr(e, P.y);
+    fe_sqr(l, e);
+    fe_add(t, P.x, e);
+    fe_sqr(t, t);
+    fe_sub(t, t, b);
+    fe_sub(t, t, l);
+    fe_add(cx, t, t);          // S
+    fe m;
+    fe_add(m, b, b);
+    fe_add(m, m, b);           // M = 3x^2 (a = 0)
+    fe_sqr(t, m);
+    fe_sub(t, t, cx);
+    fe_sub(qx, t, cx);         // X(2P) = M^2 - 2S
+    fe_sub(t, cx, qx);
+    fe_mul(t, m, t);
+    fe_add(cy, l, l);
+    fe_add(cy, cy, cy);
+    fe_add(cy, cy, cy);        // 8L
+    fe_sub(qy, t, cy);         // Y(2P) = M(S - X(2P)) - 8L
     store_aff(slot, aff{cx, cy});
     fe zlast;
     for (int j = 1; j < kPEntries; ++j) {
-        // (cx,cy,cz) += (D.x, D.y) -- madd-2007-bl; no exceptional case can
-        // arise: (2j-1)P = +-2P is impossible for a point of order n.
-        fe z1z1, u2, s2, h, hh, i4, jj, rr, v, t;
-        fe_sqr(z1z1, cz);
-        fe_mul(u2, D.x, z1z1);
-        fe_mul(s2, cz, z1z1);
-        fe_mul(s2, D.y, s2);
-        fe_sub(h, u2, cx);
-        fe_sub(rr, s2, cy);
-        fe_sqr(hh, h);
-        fe_add(i4, hh, hh);
-        fe_add(i4, i4, i4);
-        fe_mul(jj, h, i4);
-        fe_add(rr, rr, rr);
-        fe_mul(v, cx, i4);
-        fe nx, ny;
-        fe_sqr(t, rr);
-        fe_sub(t, t, jj);
-        fe_sub(t, t, v);
-        fe_sub(nx, t, v);
-        fe_sub(t, v, nx);
-        fe_mul(t, rr, t);
-        fe_mul(jj, cy, jj);
-        fe_add(jj, jj, jj);
-        fe_sub(ny, t, jj);
-        fe zr;
-        fe_add(zr, h, h);          // Z_j = Z_{j-1} * 2H
-        fe_mul(cz, cz, zr);
-        cx = nx;
-        cy = ny;
+        // (cx, cy) + (qx, qy) with a common Z; (qx, qy) moves to the new Z
+        fe dx, a, bq, bc, dy, ee;
+        fe_sub(dx, cx, qx);        // x_last - x_Q: the Z ratio
+        fe_sqr(a, dx);
+        fe_mul(bq, qx, a);         // Q's new X
+        fe_mul(bc, cx, a);
+        fe_sub(dy, cy, qy);
+        fe_sub(t, bc, bq);
+        fe_mul(ee, qy, t);         // Q's new Y = y_Q dx^3
+        fe_sqr(t, dy);
+        fe_sub(t, t, bq);
+        fe_sub(cx, t, bc);         // X = dy^2 - x_Q dx^2 - x_last dx^2
+        fe_sub(t, bq, cx);
+        fe_mul(t, dy, t);
+        fe_sub(cy, t, ee);         // Y = dy (x_Q dx^2 - X) - y_Q dx^3
+        qx = bq;
+        qy = ee;
         store_aff(slot + j, aff{cx, cy});
-        if (j < kPEntries - 1) store_fe(slot_zr(slot, j), zr);
-        zlast = zr;
+        if (j < kPEntries - 1) store_fe(slot_zr(slot, j), dx);
+        zlast = dx;
     }
-    // Rescale entries 0..6 to the common Z = cz: entry j gets the product of
-    // the ratios after it, s = zr[j+1] * ... * zr[7].
+    // Rescale entries 0..6 to the common Z of entry 7: entry j gets the
+    // product of the ratios after it, s = zr[j+1] * ... * zr[7].
     fe s = zlast;
     for (int j = kPEntries - 2; j >= 0; --j) {
         fe s2, s3;
-        aff e;
-        load_aff(e, slot + j);
+        aff en;
+        load_aff(en, slot + j);
         fe_sqr(s2, s);
         fe_mul(s3, s2, s);
-        fe_mul(e.x, e.x, s2);
-        fe_mul(e.y, e.y, s3);
-        store_aff(slot + j, e);
+        fe_mul(en.x, en.x, s2);
+        fe_mul(en.y, en.y, s3);
+        store_aff(slot + j, en);
         if (j > 0) {
             fe zr;
             load_fe(zr, slot_zr(slot, j));
             fe_mul(s, s, zr);
         }
     }
-    fe zs;
-    fe_mul(zs, cz, D.z);
+    // the common Z: 2y times every ratio (s now holds zr[1] * ... * zr[7])
+    fe zs, y2;
+    fe_add(y2, P.y, P.y);
+    fe_mul(zs, s, y2);
     store_fe(&slot[8].x, zs);
 }
 

@@ -225,12 +225,54 @@ constexpr u32 kEmpty = 0xFFFFFFFFu;
 // partial continues into the next chunk); read as an empty slot by k_level.
 constexpr u32 kCont = 0xFFFFFFFEu;
 
-__global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p, u32 n) {
+// A bucket sum or partial as the accumulators hold it: 10 x 26-bit limbs, not
+// normalised (x, y, z, then the infinity flag), 128 bytes.  Storing it takes
+// eight 16-byte stores.  Normalising to 8 x 32 limbs at every bucket end
+// would cost about as much as a quarter of a mixed addition, and in the
+// first stage some lane of a wave ends a bucket at most steps, so the whole
+// wave would pay it nearly every step.  Every reader adds these values with
+// jac26_add, whose limb bounds allow any output of the doubling, the mixed
+// and the general addition as either operand.
+struct __align__(16) jac26s { u32 w[32]; };
+
+UFS_DEV void store_j26(jac26s* p, const jac26& a) {
+    uint4* q = reinterpret_cast<uint4*>(p->w);
+    q[0] = make_uint4(a.x.v[0], a.x.v[1], a.x.v[2], a.x.v[3]);
+    q[1] = make_uint4(a.x.v[4], a.x.v[5], a.x.v[6], a.x.v[7]);
+    q[2] = make_uint4(a.x.v[8], a.x.v[9], a.y.v[0], a.y.v[1]);
+    q[3] = make_uint4(a.y.v[2], a.y.v[3], a.y.v[4], a.y.v[5]);
+    q[4] = make_uint4(a.y.v[6], a.y.v[7], a.y.v[8], a.y.v[9]);
+    q[5] = make_uint4(a.z.v[0], a.z.v[1], a.z.v[2], a.z.v[3]);
+    q[6] = make_uint4(a.z.v[4], a.z.v[5], a.z.v[6], a.z.v[7]);
+    q[7] = make_uint4(a.z.v[8], a.z.v[9], a.inf, 0u);
+}
+
+UFS_DEV void load_j26(jac26& a, const jac26s* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p->w);
+    u32 w[32];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint4 v = q[i];
+        w[4 * i] = v.x;
+        w[4 * i + 1] = v.y;
+        w[4 * i + 2] = v.z;
+        w[4 * i + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        a.x.v[i] = w[i];
+        a.y.v[i] = w[10 + i];
+        a.z.v[i] = w[20 + i];
+    }
+    a.inf = w[30];
+}
+
+__global__ void __launch_bounds__(kMsmBlock) k_j26_fill_inf(jac26s* __restrict__ p, u32 n) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    jac v;
-    jac_set_inf(v);
-    p[i] = v;
+    jac26 v;
+    jac26_set_inf(v);
+    store_j26(p + i, v);
 }
 
 // Bucket accumulation, level 0: the bucket-sorted entries are cut into chunks
@@ -242,13 +284,13 @@ __global__ void __launch_bounds__(kMsmBlock) k_jac_fill_inf(jac* __restrict__ p,
 // bucket sizes (the top window's buckets can hold half of all terms).
 __global__ void __launch_bounds__(kMsmBlock)
 k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 K, u32 max_chunks,
-       jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
+       jac26s* __restrict__ bsum, u32* __restrict__ okeys, jac26s* __restrict__ opart) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= max_chunks) return;
     const u32 beg = j * K;
-    jac hv, tv;
-    jac_set_inf(hv);
-    jac_set_inf(tv);
+    jac26 hv, tv;
+    jac26_set_inf(hv);
+    jac26_set_inf(tv);
     u32 hk = kEmpty, tk = kEmpty;
     if (beg < E) {
         const u32 end = min(beg + K, E);
@@ -257,7 +299,7 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
         const bool after = end < E && (u32)(entries[end] >> 32) == (u32)(entries[end - 1] >> 32);
         u32 cur = (u32)(ent >> 32);
         bool first = true;
-        // the running sum on 10 x 26-bit limbs, converted when a bucket ends
+        // the running sum on 10 x 26-bit limbs, stored as it is when a bucket ends
         jac26 acc;
         jac26_set_inf(acc);
         aff nxt = pts[(u32)ent & kEntryTerm];
@@ -271,9 +313,7 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
                 nxt = pts[(u32)ent & kEntryTerm];
             }
             if (k != cur) {
-                jac a8;
-                jac26_to_jac(a8, acc);
-                if (first && before) { hk = cur; hv = a8; } else { bsum[cur] = a8; }
+                if (first && before) { hk = cur; hv = acc; } else { store_j26(bsum + cur, acc); }
                 first = false;
                 cur = k;
                 jac26_set_inf(acc);
@@ -294,23 +334,21 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
             }
             acc = t;
         }
-        jac a8;
-        jac26_to_jac(a8, acc);
         if (first && before) {
             hk = cur;
-            hv = a8;
+            hv = acc;
             if (after) tk = kCont;   // the whole chunk is inside one run
         } else if (after) {
             tk = cur;
-            tv = a8;
+            tv = acc;
         } else {
-            bsum[cur] = a8;
+            store_j26(bsum + cur, acc);
         }
     }
     okeys[2 * j] = hk;
-    opart[2 * j] = hv;
+    store_j26(opart + 2 * j, hv);
     okeys[2 * j + 1] = tk;
-    opart[2 * j + 1] = tv;
+    store_j26(opart + 2 * j + 1, tv);
 }
 
 // Join of level-0 partials: a bucket that continues past its first chunk j
@@ -327,7 +365,7 @@ k_seg0(const aff* __restrict__ pts, const u64* __restrict__ entries, u32 E, u32 
 constexpr u32 kJoinSpan = 8;
 
 __global__ void __launch_bounds__(kMsmBlock)
-k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __restrict__ bsum) {
+k_join(u32* __restrict__ keys, const jac26s* __restrict__ part, u32 chunks, jac26s* __restrict__ bsum) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= chunks) return;
     const u32 tk = __hip_atomic_load(&keys[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -339,14 +377,12 @@ k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __
     }
     if (m > kJoinSpan || j + m >= chunks) return;
     jac26 acc, b;
-    jac26_from_jac(acc, part[2 * j + 1]);
+    load_j26(acc, part + 2 * j + 1);
     for (u32 i = 1; i <= m; ++i) {
-        jac26_from_jac(b, part[2 * (j + i)]);
+        load_j26(b, part + 2 * (j + i));
         jac26_add(acc, acc, b);
     }
-    jac r;
-    jac26_to_jac(r, acc);
-    bsum[tk] = r;
+    store_j26(bsum + tk, acc);
     __hip_atomic_store(&keys[2 * j + 1], kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (u32 i = 1; i <= m; ++i) keys[2 * (j + i)] = kEmpty;
 }
@@ -361,8 +397,8 @@ k_join(u32* __restrict__ keys, const jac* __restrict__ part, u32 chunks, jac* __
 // its first and last partial at most two slots away, and a level costs one
 // launch whatever is left; with nothing left every lane only reads its keys.
 __global__ void __launch_bounds__(kMsmBlock)
-k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32 K, u32 max_chunks,
-        jac* __restrict__ bsum, u32* __restrict__ okeys, jac* __restrict__ opart) {
+k_level(const u32* __restrict__ ikeys, const jac26s* __restrict__ ipart, u32 n, u32 K, u32 max_chunks,
+        jac26s* __restrict__ bsum, u32* __restrict__ okeys, jac26s* __restrict__ opart) {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= max_chunks) return;
     const u32 beg = j * K;
@@ -374,9 +410,9 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
             l = e;
         }
     u32 hk = kEmpty, tk = kEmpty;
-    jac hv, tv;
-    jac_set_inf(hv);
-    jac_set_inf(tv);
+    jac26 hv, tv;
+    jac26_set_inf(hv);
+    jac26_set_inf(tv);
     if (f < end) {
         const u32 kf = ikeys[f], kl = ikeys[l];
         // the previous partial of kf's bucket, if any, is at f - 1 or f - 2
@@ -400,27 +436,23 @@ k_level(const u32* __restrict__ ikeys, const jac* __restrict__ ipart, u32 n, u32
             const u32 k = ikeys[e];
             if (k >= kCont) continue;
             if (k != cur) {
-                jac a8;
-                jac26_to_jac(a8, acc);
-                if (first && before) { hk = cur; hv = a8; } else { bsum[cur] = a8; }
+                if (first && before) { hk = cur; hv = acc; } else { store_j26(bsum + cur, acc); }
                 first = false;
                 cur = k;
                 jac26_set_inf(acc);
             }
             jac26 b;
-            jac26_from_jac(b, ipart[e]);
+            load_j26(b, ipart + e);
             jac26_add(acc, acc, b);
         }
-        jac a8;
-        jac26_to_jac(a8, acc);
-        if (first && before) { hk = cur; hv = a8; }
-        else if (after) { tk = cur; tv = a8; }
-        else { bsum[cur] = a8; }
+        if (first && before) { hk = cur; hv = acc; }
+        else if (after) { tk = cur; tv = acc; }
+        else { store_j26(bsum + cur, acc); }
     }
     okeys[2 * j] = hk;
-    opart[2 * j] = hv;
+    store_j26(opart + 2 * j, hv);
     okeys[2 * j + 1] = tk;
-    opart[2 * j + 1] = tv;
+    store_j26(opart + 2 * j + 1, tv);
 }
 
 // r = m * a for a small non-negative integer m (double-and-add).
@@ -446,20 +478,20 @@ UFS_DEV void jac26_mul_small(jac26& r, const jac26& a, u32 m) {
 // windows' sums run beside the low windows' bucket accumulation (a second
 // stream), the short chains go first on a shared SIMD.
 __global__ void __launch_bounds__(kMsmBlock)
-k_msm_segments(const jac* __restrict__ bsum, jac* __restrict__ seg, u32 w0, u32 Gn, u32 gtop, u32 B, u32 L) {
+k_msm_segments(const jac26s* __restrict__ bsum, jac* __restrict__ seg, u32 w0, u32 Gn, u32 gtop, u32 B, u32 L) {
     __builtin_amdgcn_s_setprio(3);
     const u32 T = B / L;
     const u32 id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= Gn * T) return;
     const u32 w = w0 + id / T, s = id % T;
     const u32 off = s * L + (w == gtop ? B : 0u);
-    const jac* bb = bsum + (size_t)w * B + (size_t)s * L;
+    const jac26s* bb = bsum + (size_t)w * B + (size_t)s * L;
     jac26 R, S;
     jac26_set_inf(R);
     jac26_set_inf(S);
     for (int j = (int)L - 1; j >= 0; --j) {
         jac26 Bj;
-        jac26_from_jac(Bj, bb[j]);
+        load_j26(Bj, bb + j);
         jac26_add(R, R, Bj);
         jac26_add(S, S, R);
     }
@@ -605,7 +637,7 @@ hipError_t launch_msm_segments(const void* bsum, void* seg, const MsmPlan& p, ui
     const uint32_t T = (uint32_t)(p.buckets / p.seg_len);
     const uint32_t lanes = gn * T;
     hipLaunchKernelGGL(k_msm_segments, dim3(blocks_for(lanes, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                       (const jac*)bsum, (jac*)seg, w0, gn, (uint32_t)p.groups - 1u, (uint32_t)p.buckets,
+                       (const jac26s*)bsum, (jac*)seg, w0, gn, (uint32_t)p.groups - 1u, (uint32_t)p.buckets,
                        (uint32_t)p.seg_len);
     return hipGetLastError();
 }
@@ -805,7 +837,7 @@ GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLa
     L.slots0 = 2 * L.chunks0;
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
-    const size_t jsz = sizeof(jac), S = L.slots0;
+    const size_t jsz = sizeof(jac), bsz = sizeof(jac26s), S = L.slots0;
     // hipcub temporary storage of the bucket sort
     size_t sort_bytes = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint64_t*)nullptr,
@@ -813,8 +845,8 @@ GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLa
     if (e != hipSuccess) return be_->fail(GpuError::Launch, "MSM temporary-storage size query", e);
     L.sort_bytes = sort_bytes;
     const size_t sizes[MsmLayout::kParts] = {
-        2 * n * sizeof(aff), L.max_entries * 8, L.max_entries * 8, nbk * jsz, nseg * jsz,
-        nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S * 4, S * jsz, S * jsz, L.sort_bytes + 16};
+        2 * n * sizeof(aff), L.max_entries * 8, L.max_entries * 8, nbk * bsz, nseg * jsz,
+        nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S * 4, S * bsz, S * bsz, L.sort_bytes + 16};
     size_t off = 0;
     for (int i = 0; i < MsmLayout::kParts; ++i) {
         L.size[i] = sizes[i];
@@ -876,9 +908,9 @@ GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* con
     hipError_t e;
     const uint64_t* entries = (const uint64_t*)ptr[kEntries] + off;
     uint64_t* sorted = (uint64_t*)ptr[kSorted] + off;
-    jac* bsum = (jac*)ptr[kBsum];
+    jac26s* bsum = (jac26s*)ptr[kBsum];
     uint32_t* slot_keys[2] = {(uint32_t*)ptr[kKeys0], (uint32_t*)ptr[kKeys1]};
-    jac* slot_part[2] = {(jac*)ptr[kPart0], (jac*)ptr[kPart1]};
+    jac26s* slot_part[2] = {(jac26s*)ptr[kPart0], (jac26s*)ptr[kPart1]};
     // stable radix sort on the c low bits of the bucket id
     {
         size_t sort_bytes = L.sort_bytes;
@@ -894,7 +926,7 @@ GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* con
     // buckets spanning a few chunks (all of them for uniform scalars) are
     // completed by joining their chunk partials
     hipLaunchKernelGGL(k_join, dim3(blocks_for(chunks0, kMsmBlock)), dim3(kMsmBlock), 0, st, slot_keys[0],
-                       (const jac*)slot_part[0], (uint32_t)chunks0, bsum);
+                       (const jac26s*)slot_part[0], (uint32_t)chunks0, bsum);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM partial join", e);
     // levels over what is left, read in place (one launch each; with every
     // bucket complete a level only reads its keys), until a single chunk
@@ -904,7 +936,7 @@ GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* con
     for (;;) {
         const size_t chunks = (slots + kLevelChunk - 1) / kLevelChunk;
         hipLaunchKernelGGL(k_level, dim3(blocks_for(chunks, kMsmBlock)), dim3(kMsmBlock), 0, st,
-                           (const uint32_t*)slot_keys[lv], (const jac*)slot_part[lv], (uint32_t)slots,
+                           (const uint32_t*)slot_keys[lv], (const jac26s*)slot_part[lv], (uint32_t)slots,
                            kLevelChunk, (uint32_t)chunks, bsum, slot_keys[lv ^ 1], slot_part[lv ^ 1]);
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket level", e);
         lv ^= 1;
@@ -960,7 +992,7 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
     const unsigned cus = (unsigned)(be_->compute_units() > 0 ? be_->compute_units() : 256);
     uint32_t* err = (uint32_t*)ptr[kErr];
     void* part = (uint8_t*)ptr[kErr] + kPartOffset;
-    jac* bsum = (jac*)ptr[kBsum];
+    jac26s* bsum = (jac26s*)ptr[kBsum];
 
     // err[0]: first failing term (2i: scalar, 2i + 1: point); err[1]: bucket range flag
     if ((e = launch_u32_fill(err, 0xFFFFFFFFu, 1, st)) != hipSuccess ||
@@ -1000,7 +1032,7 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
         if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM entries", e);
     }
     // empty buckets stay at infinity
-    hipLaunchKernelGGL(k_jac_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, bsum,
+    hipLaunchKernelGGL(k_j26_fill_inf, dim3(blocks_for(nbk, kMsmBlock)), dim3(kMsmBlock), 0, st, bsum,
                        (uint32_t)nbk);
     if ((e = hipGetLastError()) != hipSuccess) return be_->fail(GpuError::Launch, "MSM bucket fill", e);
     const void* rows = nullptr;

@@ -1020,7 +1020,8 @@ GpuError MsmEngine::run_msm(const MsmLayout& L, const MsmPlan& p, void* const* p
         hipStream_t st;
         hipEvent_t ev;
         ~PointsJoin() {
-            if (ev) (void)hipStreamWaitEvent(st, ev, 0);
+            // if the stream cannot wait, the host does
+            if (ev && hipStreamWaitEvent(st, ev, 0) != hipSuccess) (void)hipEventSynchronize(ev);
         }
     } join_points{st, points};
     // unsorted window-major entries of every window (zero digits are skip entries)

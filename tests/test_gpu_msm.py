@@ -224,3 +224,39 @@ def test_msm_two_stream_form(tmp_path, monkeypatch, golden):
         k_last = int.from_bytes(bytes(ks[2047]), "big")
         sc[2047] = np.frombuffer(((-e) * pow(k_last, -1, N) % N).to_bytes(32, "big"), np.uint8)
         assert g.msm_rc(sc, pts, 2048, 11) == (UFSECP_ERR_ARITH, b"\x00" * 33)
+
+
+@pytest.mark.parametrize("c", [0, 16])
+def test_msm_invalid_points_in_a_large_batch(gpu, c):
+    """An invalid base among 2^16 terms: its term's bucket entries still go
+    through the accumulation (the bases are checked beside the scalar side,
+    not before it), and the call returns BAD_PUBKEY (3) with a zeroed output;
+    a scalar >= n at an earlier term returns BAD_INPUT (5) instead.  A valid
+    MSM on the same context afterwards is unaffected."""
+    from ec_math import P
+    n = 1 << 16
+    sc, pts, ks = wl.generate_msm(n, seed=wl.SEED_MSM + 700 + c, threads=16)
+    x_ge_p = np.frombuffer(b"\x02" + (P + 5).to_bytes(32, "big"), np.uint8)
+    off_curve = None
+    for x in range(1, 100):                   # x^3 + 7 not a square: no point with this x
+        if pow(x ** 3 + 7, (P - 1) // 2, P) != 1:
+            off_curve = np.frombuffer(b"\x03" + x.to_bytes(32, "big"), np.uint8)
+            break
+    bad_prefix = pts[n // 3].copy()
+    bad_prefix[0] = 0x04
+    for pos, bad in ((n // 2, x_ge_p), (n - 1, off_curve), (n // 3, bad_prefix), (0, x_ge_p)):
+        q = pts.copy()
+        q[pos] = bad
+        assert gpu.msm_rc(sc, q, n, c) == (3, b"\x00" * 33), pos
+        s2 = sc.copy()
+        if pos + 1 < n:
+            s2[pos + 1] = 0xFF                          # scalar >= n after the bad point
+            assert gpu.msm_rc(s2, q, n, c)[0] == 3, pos
+        s2 = sc.copy()
+        s2[pos] = 0xFF                                  # at the same term: the scalar is checked first
+        assert gpu.msm_rc(s2, q, n, c)[0] == 5, pos
+        if pos > 0:
+            s3 = sc.copy()
+            s3[pos - 1] = 0xFF                          # before it
+            assert gpu.msm_rc(s3, q, n, c)[0] == 5, pos
+    assert gpu.msm_rc(sc, pts, n, c) == (UFSECP_OK, compress(mul(wl.msm_closed_form_scalar(sc, ks))))

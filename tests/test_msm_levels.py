@@ -154,11 +154,15 @@ def bucket_sums(keys, vals, skip, K0, SPAN, K):
     bsum = Sums()
     slot_keys, slot_part, chunks0 = seg0(keys, vals, skip, K0, bsum)
     slots = 2 * chunks0
+    # the two slot buffers as the host sizes them: the first stage's slots,
+    # and two per chunk of the first level; levels alternate between them
+    cap = [slots, 2 * ((slots + K - 1) // K)]
     slot_keys, slot_part = join(slot_keys, slot_part, chunks0, SPAN, bsum)
     levels = 0
     while True:
         chunks = (slots + K - 1) // K
         slot_keys, slot_part, _ = level(slot_keys, slot_part, slots, K, bsum)
+        assert 2 * chunks <= cap[(levels + 1) % 2], "a level's output overflows its buffer"
         levels += 1
         if chunks == 1:
             break

@@ -13,8 +13,7 @@ namespace ufs {
 
 constexpr int kMsmBlock = 256;
 constexpr int kReduceBlock = 256;
-constexpr uint32_t kSegChunk = 32;   // entries per lane in the bucket reduction (and the layout's bound)
-constexpr uint32_t kSegChunkWide = 64;   // ... once that leaves at least two rounds of resident lanes
+constexpr uint32_t kSegChunkMax = 64;   // entries per lane in the first accumulation stage: 32 or 64
 constexpr uint32_t kLevelChunk = 16;  // slots per lane in the levels after the joins
 // a level writes two slots per chunk: chunks of four or more slots shrink
 // every slot count above one chunk (with three, four slots stay four)
@@ -680,8 +679,26 @@ MsmPlan msm_plan(size_t n, int c, const Tuning* tun) {
     p.windows = (kScalarBits + c - 1) / c;
     p.groups = p.windows + 1;
     p.buckets = 1 << (c - 1);
+    // Segments of 8 buckets, shorter (down to 2) while that leaves fewer than
+    // 16384 lanes for the window sums: their chains of point additions are
+    // latency-bound, so small bucket counts want more, shorter chains
+    // (measured 0.75 -> 0.68 ms at 2^10 terms, 0.78 -> 0.72 at 2^12, 0.88 ->
+    // 0.83 at 2^14 and 2^16; 2^18 and up keep 8).
     p.seg_len = p.buckets < 8 ? p.buckets : 8;
+    while (p.seg_len > 2 && (size_t)p.groups * p.buckets / p.seg_len < 16384) p.seg_len /= 2;
     return p;
+}
+
+// Entries per lane of k_seg0 for E entries on a GPU of `cus` compute units.
+// The stage runs two waves per SIMD (512 lanes per CU).  Chunks of 64 halve
+// the partials to join but pay only while the grid fills the GPU twice over
+// (2^20 terms at c = 16 measured 3.26 -> 3.15 ms; at 2^18 and 2^16 the wider
+// chunks were slower).  Narrower chunks than 32 for small E (8 or 16, so
+// that one round of resident lanes is filled) were faster at 2^10 and 2^14
+// terms but slower at 2^12 and 2^16 (1.05 and 1.46 against 0.78 and 0.88
+// ms): more partials for the join and the levels.
+static uint32_t msm_seg_chunk(size_t E, size_t cus) {
+    return E >= (size_t)kSegChunkMax * 2 * cus * 512 ? kSegChunkMax : kSegChunkMax / 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -755,6 +772,10 @@ GpuError MsmEngine::msm_device_noalloc(const uint8_t* d_scalars32, const uint8_t
     MsmLayout L;
     GpuError r = msm_layout(n, p, st, L);
     if (r != GpuError::Ok) return r;
+    // wider first-stage chunks need fewer partial slots: take the narrowest
+    // that fits the scratch
+    for (uint32_t K = L.seg_chunk * 2; L.total > kNoAllocScratchBytes && K <= kSegChunkMax; K *= 2)
+        if ((r = msm_layout(n, p, st, L, K)) != GpuError::Ok) return r;
     if (L.total > kNoAllocScratchBytes)
         return be_->fail(GpuError::BadInput,
                          "no-allocation MSM: this term count and window width need more than the 32 MiB scratch "
@@ -828,13 +849,20 @@ GpuError MsmEngine::enqueue(const uint8_t* d_scalars32, const uint8_t* d_points,
 // Byte layout of one MSM's device buffers for n terms at plan p; the
 // hipcub temporary-storage sizes are queried for this n (host only, no
 // allocation).  Offsets are 256-byte aligned from a common base.
-GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L) {
+GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L, uint32_t seg_chunk) {
     L = MsmLayout{};
     L.max_entries = 2 * n * (size_t)p.windows;   // two GLV terms per term
     if (L.max_entries > 0x7FFFFFFFu)
         return be_->fail(GpuError::BadInput, "MSM: too many (term, window) pairs for this window width");
-    L.chunks0 = (L.max_entries + kSegChunk - 1) / kSegChunk;
+    const size_t cus = (size_t)(be_->compute_units() > 0 ? be_->compute_units() : 256);
+    L.seg_chunk = seg_chunk ? seg_chunk : msm_seg_chunk(L.max_entries, cus);
+    // every bucket phase covers at most max_entries with this chunk width
+    L.chunks0 = (L.max_entries + L.seg_chunk - 1) / L.seg_chunk;
     L.slots0 = 2 * L.chunks0;
+    // the levels write two slots per chunk of kLevelChunk: the second slot
+    // buffer holds level 1's output, and every later level's fits the buffer
+    // it alternates into
+    const size_t S1 = 2 * ((L.slots0 + kLevelChunk - 1) / kLevelChunk);
     const size_t nbk = (size_t)p.groups * p.buckets;
     const size_t nseg = (size_t)p.groups * (p.buckets / p.seg_len);
     const size_t jsz = sizeof(jac), bsz = sizeof(jac26s), S = L.slots0;
@@ -846,7 +874,7 @@ GpuError MsmEngine::msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLa
     L.sort_bytes = sort_bytes;
     const size_t sizes[MsmLayout::kParts] = {
         2 * n * sizeof(aff), L.max_entries * 8, L.max_entries * 8, nbk * bsz, nseg * jsz,
-        nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S * 4, S * bsz, S * bsz, L.sort_bytes + 16};
+        nseg * jsz, nseg * jsz, 64 + jsz, S * 4, S1 * 4, S * bsz, S1 * bsz, L.sort_bytes + 16};
     size_t off = 0;
     for (int i = 0; i < MsmLayout::kParts; ++i) {
         L.size[i] = sizes[i];
@@ -898,12 +926,7 @@ GpuError MsmEngine::bucket_phase(const MsmLayout& L, const MsmPlan& p, void* con
                                  hipEvent_t points, hipStream_t st) {
     const size_t n2 = 2 * n;
     const size_t off = (size_t)wa * n2, E = (size_t)(wb - wa) * n2;
-    // k_seg0 runs two waves per SIMD: 512 lanes per CU.  Chunks of 64 halve
-    // the partials to join and the slots to scan, but only pay while the grid
-    // still fills the GPU twice over (2^20 terms at c = 16 measured 3.26 ->
-    // 3.15 ms; at 2^18 and 2^16 the wider chunks were slower).
-    const size_t cus = (size_t)(be_->compute_units() > 0 ? be_->compute_units() : 256);
-    const uint32_t K = E >= (size_t)kSegChunkWide * 2 * cus * 512 ? kSegChunkWide : kSegChunk;
+    const uint32_t K = L.seg_chunk;
     const size_t chunks0 = (E + K - 1) / K, slots0 = 2 * chunks0;
     hipError_t e;
     const uint64_t* entries = (const uint64_t*)ptr[kEntries] + off;

@@ -68,6 +68,7 @@ struct MsmLayout {
     static constexpr int kParts = 13;
     size_t off[kParts], size[kParts];
     size_t total, max_entries, chunks0, slots0, sort_bytes;
+    uint32_t seg_chunk;   // entries per lane of the first accumulation stage
 };
 
 class MsmEngine {
@@ -101,7 +102,8 @@ private:
                      int32_t* d_status, int c, hipStream_t st);
     GpuError enqueue_work(const uint8_t* d_scalars32, const uint8_t* d_points, bool affine, size_t n,
                           uint8_t* d_out33, int32_t* d_status, int c, hipStream_t st);
-    GpuError msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L);
+    // seg_chunk 0: the width for this size (msm_seg_chunk)
+    GpuError msm_layout(size_t n, const MsmPlan& p, hipStream_t st, MsmLayout& L, uint32_t seg_chunk = 0);
     // split: windows [W/2, W) go first and their sums and Horner share run
     // on the engine's second stream beside the low windows' bucket phase
     GpuError run_msm(const MsmLayout& L, const MsmPlan& p, void* const* ptr, const uint8_t* d_scalars32,

@@ -260,3 +260,34 @@ def test_msm_invalid_points_in_a_large_batch(gpu, c):
             s3[pos - 1] = 0xFF                          # before it
             assert gpu.msm_rc(s3, q, n, c)[0] == 5, pos
     assert gpu.msm_rc(sc, pts, n, c) == (UFSECP_OK, compress(mul(wl.msm_closed_form_scalar(sc, ks))))
+
+
+def test_msm_random_shapes(gpu):
+    """Seeded random MSMs: sizes log-uniform in [1, 2^15], explicit and tuned
+    window widths, and scalar patterns that stress the bucket stages
+    (duplicated scalars, half zero, small scalars, one scalar for every term,
+    repeated bases), against the closed form (sum s_i k_i) G."""
+    import random
+    rnd = random.Random(20261204)
+    base_sc, base_pts, base_ks = wl.generate_msm(1 << 15, seed=4242, threads=16)
+    for it in range(40):
+        n = max(1, min(int(round(2 ** rnd.uniform(0, 15))), 1 << 15))
+        c = rnd.choice([0, 0, 0, 2, 3, 5, 8, 11, 13, 16, 18, 20])
+        off = rnd.randrange(0, (1 << 15) - n + 1)
+        sc, pts, ks = base_sc[off:off + n].copy(), base_pts[off:off + n].copy(), base_ks[off:off + n].copy()
+        pat = rnd.choice(["uniform", "dup", "zeros", "small", "onebucket", "rep_points"])
+        if pat == "dup":
+            sc = sc[[rnd.randrange(n) for _ in range(n)]]
+        elif pat == "zeros":
+            sc[np.array([rnd.random() < 0.5 for _ in range(n)])] = 0
+        elif pat == "small":
+            for i in range(n):
+                sc[i] = np.frombuffer(rnd.randrange(0, 1 << rnd.choice([1, 8, 20, 40])).to_bytes(32, "big"), np.uint8)
+        elif pat == "onebucket":
+            sc[:] = sc[0]
+        elif pat == "rep_points":
+            j = [rnd.randrange(max(1, n // 8)) for _ in range(n)]
+            pts, ks = pts[j], ks[j]
+        e = wl.msm_closed_form_scalar(sc, ks)
+        want = (UFSECP_ERR_ARITH, b"\x00" * 33) if e == 0 else (UFSECP_OK, compress(mul(e)))
+        assert gpu.msm_rc(sc, pts, n, c) == want, (it, n, c, pat, off)

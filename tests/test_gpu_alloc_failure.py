@@ -76,7 +76,11 @@ def _sweep(call, check):
     return made
 
 
-def test_context_creation_fails_cleanly_at_each_allocation():
+def test_context_creation_fails_cleanly_at_each_allocation(monkeypatch):
+    # a comb width no other context of the process holds, so that this
+    # context allocates its own G table (the table is shared per device and
+    # width)
+    monkeypatch.setenv("UFSECP_GCOMB_W", "11")
     lib = load_library()
     before = _count(lib)
     UfsecpGpu().close()
@@ -93,6 +97,28 @@ def test_context_creation_fails_cleanly_at_each_allocation():
     b = wl.generate("schnorr", 1000)
     with UfsecpGpu() as g:
         assert np.array_equal(g.schnorr_verify_batch(b.msgs, b.pubkeys, b.sigs), 1 - b.corrupted)
+
+
+def test_contexts_share_the_g_table(monkeypatch):
+    """A second context on the device with the same comb width takes the
+    first one's G table (one allocation fewer), both verify exactly, and the
+    table is freed with the last context that holds it."""
+    monkeypatch.setenv("UFSECP_GCOMB_W", "12")
+    lib = load_library()
+    live0 = _live(lib)
+    b = wl.generate("ecdsa", 700)
+    c0 = _count(lib)
+    a = UfsecpGpu()
+    first = _count(lib) - c0
+    c1 = _count(lib)
+    second = UfsecpGpu()
+    assert _count(lib) - c1 == first - 1
+    for g in (a, second):
+        assert np.array_equal(g.ecdsa_verify_batch(b.msgs, b.pubkeys, b.sigs), 1 - b.corrupted)
+    a.close()
+    assert np.array_equal(second.ecdsa_verify_batch(b.msgs, b.pubkeys, b.sigs), 1 - b.corrupted)
+    second.close()
+    assert _live(lib) == live0
 
 
 @pytest.mark.parametrize("kind", ["ecdsa", "schnorr"])

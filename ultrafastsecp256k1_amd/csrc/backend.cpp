@@ -2,6 +2,8 @@
 #include "backend.hpp"
 
 #include <atomic>
+#include <mutex>
+#include <vector>
 
 #include <cstdio>
 #include <cstdlib>
@@ -128,6 +130,59 @@ GpuError RocmBackend::fail(GpuError e, const char* what, hipError_t he) {
     return e;
 }
 
+// The fixed-base comb for G is the same for every context on a device (436 MB
+// at width 20): one table per (device, width), built by the first context
+// that needs it, shared read-only, freed with the last context using it.
+namespace {
+struct SharedComb {
+    int device;
+    int w;
+    void* ptr;
+    int refs;
+};
+std::mutex g_comb_mu;
+std::vector<SharedComb> g_combs;
+}  // namespace
+
+// The caller's device is current; the build finishes before the table is
+// handed to any context.
+// *alloc_failed tells an allocation failure from a failed build.
+static hipError_t acquire_comb(int device, int w, hipStream_t st, void** out, bool* alloc_failed) {
+    std::lock_guard<std::mutex> lk(g_comb_mu);
+    for (SharedComb& c : g_combs)
+        if (c.device == device && c.w == w) {
+            ++c.refs;
+            *out = c.ptr;
+            return hipSuccess;
+        }
+    void* p = nullptr;
+    hipError_t e = dev_malloc(&p, gcomb_entries(w) * 64);
+    *alloc_failed = e != hipSuccess;
+    if (e != hipSuccess) return e;
+    e = launch_build_gcomb(p, w, gcomb_nwin(w), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)dev_free(p);
+        return e;
+    }
+    g_combs.push_back(SharedComb{device, w, p, 1});
+    *out = p;
+    return hipSuccess;
+}
+
+static void release_comb(int device, void* ptr) {
+    if (!ptr) return;
+    std::lock_guard<std::mutex> lk(g_comb_mu);
+    for (size_t i = 0; i < g_combs.size(); ++i)
+        if (g_combs[i].device == device && g_combs[i].ptr == ptr) {
+            if (--g_combs[i].refs == 0) {
+                (void)dev_free(ptr);
+                g_combs.erase(g_combs.begin() + (std::ptrdiff_t)i);
+            }
+            return;
+        }
+}
+
 RocmBackend::~RocmBackend() {
     if (device_ >= 0) (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
@@ -146,7 +201,7 @@ RocmBackend::~RocmBackend() {
     h_tables_.release();
     for (auto& b : d_in_) b.release();
     d_out_.release();
-    (void)dev_free(d_gcomb_);
+    release_comb(device_, d_gcomb_);
     (void)dev_free(d_mid_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -170,14 +225,14 @@ GpuError RocmBackend::init(uint32_t device_index) {
     if (e != hipSuccess) return fail(GpuError::Queue, "hipStreamCreate", e);
 
     // Fixed-base comb table for G: width from the tuning file, overridden by
-    // UFSECP_GCOMB_W (4..20).
+    // UFSECP_GCOMB_W (4..20); shared by the contexts on this device.
     tuning_ = load_tuning();
     const int w = env_int("UFSECP_GCOMB_W", tuning_.gcomb_w, 4, 20);
-    const size_t entries = gcomb_entries(w);
-    e = dev_malloc(&d_gcomb_, entries * 64);
-    if (e != hipSuccess) return fail(GpuError::Memory, "hipMalloc(G table)", e);
-    e = launch_build_gcomb(d_gcomb_, w, gcomb_nwin(w), stream_);
-    if (e != hipSuccess) return fail(GpuError::Launch, "build G table", e);
+    bool alloc_failed = false;
+    e = acquire_comb(device_, w, stream_, &d_gcomb_, &alloc_failed);
+    if (e != hipSuccess)
+        return fail(alloc_failed ? GpuError::Memory : GpuError::Launch,
+                    alloc_failed ? "hipMalloc(G table)" : "build G table", e);
     gdesc_ = GcombDesc{d_gcomb_, w, gcomb_nwin(w)};
 
     e = dev_malloc(&d_mid_, 8 * sizeof(uint32_t));

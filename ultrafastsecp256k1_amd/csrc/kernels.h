@@ -94,7 +94,7 @@ struct TileDesc {
 constexpr uint32_t kTileItems = 64;
 
 // Bytes of lane-private table space the verify kernel needs per grid lane
-// (twelve 64-byte affine entries).
+// (thirteen 64-byte entries: the table, its Z scale and ratios, a stash).
 constexpr size_t kSlotBytes = 832;
 
 // Arguments of the persistent verify kernel.  Single mode (tiles == nullptr):

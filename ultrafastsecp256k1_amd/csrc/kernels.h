@@ -98,7 +98,8 @@ constexpr uint32_t kTileItems = 64;
 constexpr size_t kSlotBytes = 832;
 
 // Arguments of the persistent verify kernel.  Single mode (tiles == nullptr):
-// the tiles of `single`, single_tiles of them.  List mode: tiles[0 .. *ntiles_dev)
+// the tiles of `single`, single_tiles of them, taken through `counter` when it
+// is set (zero at launch), else by a fixed stride.  List mode: tiles[0 .. *ntiles_dev)
 // over descs, distributed through `counter`, which must read zero at launch
 // (ntiles_dev and counter are device words so a captured graph stays valid
 // when the tile list changes between replays).

@@ -355,14 +355,26 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) mid[j] = mid_g[j];
     const bool list = a.tiles != nullptr;
+    // single mode with a work counter: waves take the next tile (two
+    // consecutive tiles when pairing) as they finish instead of a fixed
+    // stride, so waves that run slower take fewer tiles (2^20 items: about 6%
+    // faster than the stride)
+    const bool dyn = !list && a.counter != nullptr;
     const u32 ntiles = list ? *a.ntiles_dev : a.single_tiles;
     const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
     u32 t = gtid >> 6;
+    // single ECDSA or BIP-340 batch: the lane's item of the pair's second
+    // tile joins the one of tile t (shared inversion)
+    const bool pairing = (KIND == 0 || KIND == 1) && !list;
     for (;;) {
-        if (list) {
+        u32 t2;
+        if (list || dyn) {
             u32 x = 0;
-            if (lane == 0) x = atomicAdd(a.counter, 1u);
+            if (lane == 0) x = atomicAdd(a.counter, pairing ? 2u : 1u);
             t = __shfl(x, 0, 64);
+            t2 = t + 1;
+        } else {
+            t2 = t + nwaves;
         }
         if (t >= ntiles) break;
         BatchDesc d;
@@ -381,11 +393,9 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
             d = a.single;
             first = t * kTileItems;
         }
-        // single ECDSA or BIP-340 batch: the lane's item of tile t + W joins
-        // the one of tile t (shared inversion); the tile loop then steps by 2W
-        const bool pairing = (KIND == 0 || KIND == 1) && !list;
-        const u32 t2 = t + nwaves;
-        t += pairing ? 2 * nwaves : nwaves;
+        // without a counter the tiles go by a fixed stride: t, t + W (pairs:
+        // the loop then steps by 2W)
+        if (!list && !dyn) t += pairing ? 2 * nwaves : nwaves;
         const u32 i = first + lane;
         if (i >= d.count) continue;
         const u32 i2 = t2 * kTileItems + lane;

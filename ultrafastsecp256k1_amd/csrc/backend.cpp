@@ -298,11 +298,13 @@ hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStrea
     a.slots = d_slots_.ptr;
     a.single_tiles = tiles;
     a.collect = collect ? 1u : 0u;
-    // the waves take tiles from a work counter, zeroed on st before the launch
-    // (the verify launches of a context are ordered, so they share the word)
-    a.counter = (uint32_t*)d_words_.ptr + 48;
+    // more tiles than waves: the waves take tiles from a work counter, zeroed
+    // on st before the launch (the verify launches of a context are ordered,
+    // so they share the word); otherwise one tile per wave by position
+    const bool dyn = tiles > (size_t)grid * waves_per_block;
+    if (dyn) a.counter = (uint32_t*)d_words_.ptr + 48;
     hipError_t e = verify_begin(st);
-    if (e == hipSuccess) e = launch_u32_fill(a.counter, 0u, 1, st);
+    if (e == hipSuccess && dyn) e = launch_u32_fill(a.counter, 0u, 1, st);
     if (e == hipSuccess) e = launch_verify(kind, a, gdesc_, d_mid_, grid, st);
     if (e == hipSuccess) e = verify_end(st);
     return e;

@@ -363,9 +363,10 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
     const u32 ntiles = list ? *a.ntiles_dev : a.single_tiles;
     const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
     u32 t = gtid >> 6;
-    // single ECDSA or BIP-340 batch: the lane's item of the pair's second
-    // tile joins the one of tile t (shared inversion)
-    const bool pairing = (KIND == 0 || KIND == 1) && !list;
+    // single ECDSA or BIP-340 batch with at least two tiles per wave: the
+    // lane's item of the pair's second tile joins the one of tile t (shared
+    // inversion); with fewer tiles pairs would only lengthen the launch
+    const bool pairing = (KIND == 0 || KIND == 1) && !list && ntiles >= 2 * nwaves;
     for (;;) {
         u32 t2;
         if (list || dyn) {

@@ -363,19 +363,33 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
     const u32 ntiles = list ? *a.ntiles_dev : a.single_tiles;
     const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
     u32 t = gtid >> 6;
-    // single ECDSA or BIP-340 batch with at least two tiles per wave: the
-    // lane's item of the pair's second tile joins the one of tile t (shared
-    // inversion); with fewer tiles pairs would only lengthen the launch
-    const bool pairing = (KIND == 0 || KIND == 1) && !list && ntiles >= 2 * nwaves;
+    // Single ECDSA or BIP-340 batch: the lane's item of a second tile joins
+    // the one of tile t (shared inversion).  With the counter, units
+    // 0 .. np - 1 are pairs of consecutive tiles and the units after them
+    // single tiles: as many full rounds of one pair per wave as the batch
+    // fills, then for the r < 2W tiles left one unit per wave (r - W pairs
+    // and the rest singles when r > W, else r singles), so the last round
+    // ends within one unit rather than a pair for some waves and nothing for
+    // others.  By stride: t with t + W, the loop stepping by 2W.
+    const bool pair_kind = (KIND == 0 || KIND == 1) && !list;
+    u32 np = 0;
+    if (pair_kind && dyn) {
+        const u32 r = ntiles % (2 * nwaves);
+        np = nwaves * (ntiles / (2 * nwaves)) + (r > nwaves ? r - nwaves : 0u);
+    }
     for (;;) {
-        u32 t2;
+        u32 t2 = 0;
+        bool paired = false;
         if (list || dyn) {
             u32 x = 0;
-            if (lane == 0) x = atomicAdd(a.counter, pairing ? 2u : 1u);
-            t = __shfl(x, 0, 64);
+            if (lane == 0) x = atomicAdd(a.counter, 1u);
+            const u32 u = __shfl(x, 0, 64);
+            paired = u < np;
+            t = paired ? 2 * u : (list ? u : np + u);
             t2 = t + 1;
         } else {
             t2 = t + nwaves;
+            paired = pair_kind;
         }
         if (t >= ntiles) break;
         BatchDesc d;
@@ -396,11 +410,11 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         }
         // without a counter the tiles go by a fixed stride: t, t + W (pairs:
         // the loop then steps by 2W)
-        if (!list && !dyn) t += pairing ? 2 * nwaves : nwaves;
+        if (!list && !dyn) t += pair_kind ? 2 * nwaves : nwaves;
         const u32 i = first + lane;
         if (i >= d.count) continue;
         const u32 i2 = t2 * kTileItems + lane;
-        const bool two = pairing && t2 < ntiles && i2 < d.count;
+        const bool two = paired && t2 < ntiles && i2 < d.count;
         const u32 kind = KIND >= 2 ? d.kind : (u32)KIND;
         bool v, v2 = false;
         if (kind == kKindSchnorr) {

@@ -288,11 +288,12 @@ hipError_t RocmBackend::verify_end(hipStream_t st) {
 hipError_t RocmBackend::launch_single(const BatchDesc& d, bool collect, hipStream_t st) {
     if (d.count == 0) return hipSuccess;
     const uint32_t tiles = (d.count + kTileItems - 1) / kTileItems;
-    // kernel variant: ECDSA-only, BIP-340-only, or kind read from the batch
-    const int kind = d.kind == kKindEcdsa ? 0 : d.kind == kKindSchnorr ? 1 : 2;
+    // kernel variant: ECDSA, BIP-340 or opaque ECDSA (its grid is ECDSA's)
+    const int kind = d.kind == kKindEcdsa ? 0 : d.kind == kKindSchnorr ? 1 : 4;
     const unsigned waves_per_block = kVerifyBlock / 64;
     unsigned grid = (tiles + waves_per_block - 1) / waves_per_block;
-    if (grid > verify_grid_[kind]) grid = verify_grid_[kind];
+    const unsigned gmax = verify_grid_[kind == 4 ? 0 : kind];
+    if (grid > gmax) grid = gmax;
     VerifyLaunch a{};
     a.single = d;
     a.slots = d_slots_.ptr;
@@ -314,7 +315,7 @@ hipError_t RocmBackend::upload_and_verify(const uint8_t* const* src, void* const
                                           const size_t* last_bytes, int nregions, const BatchDesc& whole, size_t ms,
                                           size_t ps, size_t ss, bool collect) {
     const size_t count = whole.count;
-    const int kv = whole.kind == kKindEcdsa ? 0 : whole.kind == kKindSchnorr ? 1 : 2;
+    const int kv = whole.kind == kKindSchnorr ? 1 : 0;
     const size_t round = (size_t)verify_grid_[kv] * kVerifyBlock;
     size_t a = 0, len = round;
     hipError_t e = hipSuccess;

@@ -122,8 +122,9 @@ struct VerifyLaunch {
 };
 
 // Kernel variant: 0 = ECDSA only, 1 = BIP-340 only, 2 = kind read per batch
-// (list mode, and single batches of the opaque ECDSA kind), 3 = as 2 with the
-// keys through the key cache (list mode).
+// (list mode), 3 = as 2 with the keys through the key cache (list mode),
+// 4 = single batches of the opaque ECDSA kind (the lane's items paired as
+// in 0; same grid as 0).
 hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, const uint32_t* mid, unsigned grid,
                          hipStream_t st);
 int verify_blocks_per_cu(int kind);

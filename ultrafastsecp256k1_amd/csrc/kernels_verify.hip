@@ -211,14 +211,14 @@ UFS_DEV void load_sc(sc& a, const fe* p) {
 // sharing the inversion of s (Montgomery's trick: 1/s1 = s2 / (s1 s2),
 // 1/s2 = s1 / (s1 s2)).  Both inverses wait in the slot's last entry while
 // the items run one after the other in a rolled loop (one inline copy of the
-// scalar multiplication).
+// scalar multiplication).  `opaque` as for ecdsa_verify_item.
 UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, const gcomb& G, aff* slot,
-                                const KeySrc& ks, bool& v1, bool& v2) {
+                                const KeySrc& ks, bool opaque, bool& v1, bool& v2) {
     fe* stash = &slot[kSlotEntries - 1].x;
     {
         sc s1, s2, ss, inv, w;
-        ecdsa_item_s(d, i1, false, s1);
-        if (two) ecdsa_item_s(d, i2, false, s2); else sc_set_u32(s2, 1);
+        ecdsa_item_s(d, i1, opaque, s1);
+        if (two) ecdsa_item_s(d, i2, opaque, s2); else sc_set_u32(s2, 1);
         sc_mul(ss, s1, s2);
         sc_inv(inv, ss);
         sc_mul(w, inv, s2);
@@ -232,7 +232,7 @@ UFS_DEV void ecdsa_verify_items(const BatchDesc& d, u32 i1, u32 i2, bool two, co
     for (int k = 0; k < m; ++k) {
         sc w;
         load_sc(w, stash + k);
-        const bool v = ecdsa_verify_item_w(d, k ? i2 : i1, G, slot, ks, false, w);
+        const bool v = ecdsa_verify_item_w(d, k ? i2 : i1, G, slot, ks, opaque, w);
         if (k == 0) v1 = v; else v2 = v;
     }
 }
@@ -363,7 +363,7 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
     const u32 ntiles = list ? *a.ntiles_dev : a.single_tiles;
     const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
     u32 t = gtid >> 6;
-    // Single ECDSA or BIP-340 batch: the lane's item of a second tile joins
+    // Single ECDSA (either form) or BIP-340 batch: the lane's item of a second tile joins
     // the one of tile t (shared inversion).  With the counter, units
     // 0 .. np - 1 are pairs of consecutive tiles and the units after them
     // single tiles: as many full rounds of one pair per wave as the batch
@@ -371,7 +371,7 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
     // and the rest singles when r > W, else r singles), so the last round
     // ends within one unit rather than a pair for some waves and nothing for
     // others.  By stride: t with t + W, the loop stepping by 2W.
-    const bool pair_kind = (KIND == 0 || KIND == 1) && !list;
+    const bool pair_kind = (KIND == 0 || KIND == 1 || KIND == 4) && !list;
     u32 np = 0;
     if (pair_kind && dyn) {
         const u32 r = ntiles % (2 * nwaves);
@@ -415,12 +415,12 @@ k_verify(VerifyLaunch a, gcomb G, const u32* __restrict__ mid_g) {
         if (i >= d.count) continue;
         const u32 i2 = t2 * kTileItems + lane;
         const bool two = paired && t2 < ntiles && i2 < d.count;
-        const u32 kind = KIND >= 2 ? d.kind : (u32)KIND;
+        const u32 kind = KIND == 4 ? kKindEcdsaOpaque : KIND >= 2 ? d.kind : (u32)KIND;
         bool v, v2 = false;
         if (kind == kKindSchnorr) {
             schnorr_verify_items(d, i, i2, two, mid, G, slot, ks, v, v2);
-        } else if (KIND == 0) {
-            ecdsa_verify_items(d, i, i2, two, G, slot, ks, v, v2);
+        } else if (KIND == 0 || KIND == 4) {
+            ecdsa_verify_items(d, i, i2, two, G, slot, ks, KIND == 4, v, v2);
         } else {
             v = ecdsa_verify_item(d, i, G, slot, ks, kind == kKindEcdsaOpaque);
         }
@@ -708,6 +708,7 @@ hipError_t launch_verify(int kind, const VerifyLaunch& a, const GcombDesc& g, co
     case 0: hipLaunchKernelGGL(k_verify<0>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     case 1: hipLaunchKernelGGL(k_verify<1>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     case 3: hipLaunchKernelGGL(k_verify<3>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
+    case 4: hipLaunchKernelGGL(k_verify<4>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     default: hipLaunchKernelGGL(k_verify<2>, dim3(grid), dim3(kVerifyBlock), 0, st, a, G, mid); break;
     }
     return hipGetLastError();
@@ -720,6 +721,7 @@ int verify_blocks_per_cu(int kind) {
     case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<0>, kVerifyBlock, 0); break;
     case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<1>, kVerifyBlock, 0); break;
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<3>, kVerifyBlock, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<4>, kVerifyBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_verify<2>, kVerifyBlock, 0); break;
     }
     if (e != hipSuccess || blocks <= 0) return 1;
